@@ -1,0 +1,191 @@
+"""Benchmark of the spin-dynamics hot path on one MI355X.
+
+Workload (default): the batched ALC field sweep -- muon + electron + 4 1H +
+14N (Hilbert dimension 192), 200 longitudinal fields in 0-3 T x 500 powder
+orientations = 1e5 Hermitian 192 x 192 eigensolves, integrate_decaying with
+tau = 2.197 us, synthetic couplings from a fixed seed.  One step = the whole
+sweep (H build -> eigh -> basis change -> integral -> powder accumulation)
+with inputs resident in HBM.
+
+Prints one JSON line: orientation-solves/s (whole job), per-kernel roofline
+of the dominant kernel from HIP events recorded in the timed region, and the
+FP64 oracle timed on the host cores over a stride sample of the same sweep
+(which also gives max|dP| of the GPU result on that sample).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic(kernel, n, ntimes, real_mode=True):
+    """(unit, amount per instance) for the roofline: real flops per instance
+    (complex multiply-add = 8), or HBM bytes for pure data-movement kernels."""
+    if kernel == "k_tridiag":
+        return "flop", 16.0 / 3.0 * n ** 3
+    if kernel == "k_backtransform":
+        return "flop", 8.0 * n ** 3
+    if kernel == "k_gemm_cn":
+        return "flop", 8.0 * n ** 3
+    if kernel == "k_eigh_jacobi":
+        return "flop", (16.0 / 3.0 + 8.0) * n ** 3
+    if kernel == "k_stedc":
+        return "flop", 2.0 * n ** 3
+    if kernel == "k_timesum":
+        return "flop", (4.0 if real_mode else 8.0) * n * n * ntimes
+    if kernel == "k_integral":
+        return "flop", 20.0 * n * n
+    if kernel == "k_build_h":
+        return "byte", 16.0 * n * n
+    if kernel == "k_apply_muon":
+        return "byte", 2 * 16.0 * n * n
+    if kernel == "k_form_c":
+        return "byte", 24.0 * n * n
+    return "byte", 0.0
+
+
+def workload(name):
+    from muspinsim_amd import workloads as W
+    if name == "c2":
+        return W.c2_alc_sweep(), "alc_sweep_n192_200fields_x_500orient_integral"
+    if name == "c1":
+        return W.c1_zero_field_powder(), "zero_field_powder_n16_100orient_1000t"
+    if name == "c3":
+        return W.c3_tf_stress(), "tf_stress_n512_256orient_4000t"
+    if name == "c5":
+        return W.c5_fit(), "fit_forward_n64_256orient_2000t"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-sample", type=int, default=4000,
+                    help="instances of the sweep timed on the host (0 = skip)")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(instances shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    (systems, batch), wname = workload(args.workload)
+    n = systems[0].n
+    ninst = len(batch)
+    width = 1 if batch.y == "integral" else len(batch.times)
+
+    # ---- CPU baseline first: forked workers, before anything touches the GPU
+    cpu = None
+    cpu_rows = None
+    sample = None
+    if args.cpu_sample > 0:
+        from oracle import baseline
+        stride = max(1, ninst // args.cpu_sample)
+        sample = np.arange(0, ninst, stride)[: args.cpu_sample]
+        procs = args.cpu_procs or min(16, baseline.host_cores())
+        cpu_rows, wall, busy = baseline.run(systems, batch, sample, procs)
+        one = sample[: max(1, min(len(sample), 64))]
+        _, wall1, _ = baseline.run(systems, batch, one, 1)
+        cpu = {
+            "value": len(sample) / wall,
+            "unit": "orientation-solves/s",
+            "cores": procs,
+            "kind": "port",
+            "sample": f"{len(sample)} of {ninst} instances (stride {stride}) of the same sweep, "
+                      f"numpy FP64 eigh + basis change + integral, {procs} processes x 1 BLAS "
+                      f"thread on {baseline.cpu_model()}",
+            "single_core_value": len(one) / wall1,
+        }
+
+    # ---- GPU
+    import torch
+    from muspinsim_amd import _native as N
+    from muspinsim_amd.engine import SpinEngine
+
+    eng = SpinEngine("cuda:0")
+    plan = eng.prepare(systems, batch)
+    for _ in range(args.warmup):
+        eng.execute(plan)
+    torch.cuda.synchronize()
+    N.profile_report()                      # clear
+    N.load_library().msp_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.execute(plan)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    N.load_library().msp_profile_enable(0)
+    prof = N.profile_report()
+    results = plan.results.cpu().numpy()
+    rows = plan.rows.cpu().numpy()
+
+    # untimed pass without event bracketing, to report the overhead-free rate
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.execute(plan)
+    torch.cuda.synchronize()
+    dt_clean = time.perf_counter() - t1
+
+    value = ninst * args.steps / dt
+    # dominant kernel and its roofline
+    kname = max(prof, key=lambda k: prof[k][1])
+    launches, tot_ms = prof[kname]
+    unit, per_inst = algorithmic(kname, n, width)
+    inst_per_launch = ninst * args.steps / launches
+    avg_s = tot_ms / launches / 1e3
+    if unit == "flop":
+        achieved = per_inst * inst_per_launch / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        achieved = per_inst * inst_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    roof["kernel"] = kname
+    roof["avg_launch_ms"] = tot_ms / launches
+    roof["instances_per_launch"] = inst_per_launch
+    maxdp = None
+    if cpu_rows is not None:
+        maxdp = float(np.abs(rows[sample] - cpu_rows).max())
+    line = {
+        "metric": "orientation-solves/s",
+        "value": value,
+        "unit": "orientation-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded couplings and orientations)",
+        "config": {"workload": wname, "hilbert_dim": n, "instances": ninst,
+                   "precision": "fp64-plain", "eigensolver": "householder+divide-and-conquer"
+                   if n > 64 else "jacobi-lds"},
+        "max_abs_dP_vs_fp64_oracle": maxdp,
+        "value_without_event_timing": ninst / dt_clean,
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(prof.items())},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "results_checksum": float(np.sum(results)),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

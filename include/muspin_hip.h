@@ -1,0 +1,156 @@
+/* muspin_hip.h -- C ABI of the MI355X (gfx950) spin-dynamics engine.
+ *
+ * One shared library, libmuspin_hip.so, exporting plain-pointer entry points
+ * for the muspinsim hot path: per (orientation x field) instance, build H,
+ * diagonalise it, change the density matrix and observable into the
+ * eigenbasis, and evaluate the time sum / decaying integral with the
+ * powder-weighted accumulation fused into the last stage.
+ *
+ * Conventions
+ *   - Every pointer argument named d_* is a DEVICE pointer (HIP global
+ *     memory, e.g. torch.Tensor.data_ptr()); h_* pointers are host memory.
+ *   - Complex numbers are interleaved (re, im) doubles: numpy complex128.
+ *   - Matrices are dense, row-major, n x n, batched with stride n*n.
+ *   - Calls are asynchronous on `stream` (a hipStream_t; 0 = default
+ *     stream) and return 0 on success or an MSP_ERR_* code; the message of
+ *     the last failure on the calling thread is read with msp_last_error.
+ *   - The library allocates no device memory: scratch space comes from the
+ *     caller through d_work, sized by the matching *_workspace function.
+ *
+ * Reference interfaces replaced (paths in the muspinsim tree, v2.3.1):
+ *   msp_rotate_fields   ExperimentRunner.load_config rotations, experiment.py:384-411
+ *   msp_build_h         SpinSystem.hamiltonian + ExperimentRunner.Hz/Htot,
+ *                       spinsys.py:34-69,613-626; experiment.py:238-269,331-341
+ *   msp_eigh            Hermitian.diag -> numpy.linalg.eigh, spinop.py:51-82
+ *   msp_apply_muon_op   MuonSpinSystem.muon_operator / sigma_mu (x) 1,
+ *                       spinsys.py:707-760; hamiltonian.py:204
+ *   msp_apply_kron      DensityOperator kron of per-spin states, experiment.py:169-236
+ *   msp_gemm_cn         Operator.basis_change (U^H M U), spinop.py:330-355
+ *   msp_form_c          rho' * O'^T of Hamiltonian.evolve, hamiltonian.py:77-103;
+ *                       |U^H sigma U|^2 of fast_evolve, hamiltonian.py:207-210
+ *   msp_timesum         evolve time loop, hamiltonian.py:87-107, and the
+ *                       cosine sum parallel_fast_time_evolve, cython/parallel.pyx:16-69
+ *   msp_integral        Hamiltonian.integrate_decaying, hamiltonian.py:150-162
+ *   msp_accumulate      Re(data)*w and store_time_slice powder accumulation,
+ *                       experiment.py:498; simconfig.py:347-368
+ */
+#ifndef MUSPIN_HIP_H
+#define MUSPIN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSP_OK 0
+#define MSP_ERR_VALUE 1       /* invalid argument      -> ValueError        */
+#define MSP_ERR_NOCONV 2      /* solver not converged  -> LinAlgError       */
+#define MSP_ERR_HIP 3         /* HIP runtime failure   -> RuntimeError      */
+#define MSP_ERR_UNSUPPORTED 4 /* size/feature limit    -> NotImplementedError */
+
+#define MSP_MAX_SPINS 16
+
+/* One spin system: spins in Kronecker order (first = slowest index). */
+typedef struct MspSystem {
+    int32_t n_spins;
+    int32_t n;            /* Hilbert dimension = prod(dims) */
+    int32_t muon_index;
+    int32_t term_offset;  /* first term in the term table */
+    int32_t n_terms;
+    int32_t pad[3];
+    int32_t dims[MSP_MAX_SPINS];    /* 2I+1 per spin */
+    double gammas[MSP_MAX_SPINS];   /* MHz/T */
+} MspSystem;
+
+/* One interaction term: kind 0 scalar (t[0]), 1 linear t[0..2].S_i,
+ * 2 bilinear S_i.T.S_j with T = t[0..8] row-major (i == j: same-spin
+ * product S_i^a S_i^b, as for quadrupolar couplings).  MHz. */
+typedef struct MspTerm {
+    int32_t kind;
+    int32_t i;
+    int32_t j;
+    int32_t pad;
+    double t[9];
+} MspTerm;
+
+/* Eigensolver selection for msp_eigh. */
+#define MSP_EIGH_AUTO 0
+#define MSP_EIGH_JACOBI 1   /* LDS-resident cyclic Jacobi, n <= 64 */
+#define MSP_EIGH_DC 2       /* Householder + divide and conquer, any n */
+
+int msp_init(void);
+int msp_last_error(char *h_buf, size_t len);
+int msp_version(void);
+
+/* B' = q.B + B_int and p' = q.p for each instance (q stored conjugated,
+ * [w, x, y, z]); arrays are batch x 4 / batch x 3 doubles. */
+int msp_rotate_fields(int batch, const double *d_q, const double *d_B, const double *d_Bint,
+                      const double *d_p, double *d_Br, double *d_pr, void *stream);
+
+/* H[b] = base[sys[b]] (if d_base != NULL) or the sum of the system's terms,
+ * plus the Zeeman term sum_i gamma_i Br[b].S_i.  All instances share n.
+ * zeeman = 0 skips the Zeeman part (used to build the base matrices). */
+int msp_build_h(int n, int batch, const MspSystem *d_systems, const MspTerm *d_terms,
+                const int32_t *d_sys_id, const double *d_Br, const double *d_base,
+                int zeeman, double *d_H, void *stream);
+
+/* Batched Hermitian eigendecomposition.  d_H is destroyed.  d_E: batch x n
+ * ascending eigenvalues; d_U: batch x n x n eigenvectors (columns).
+ * d_info: batch int32, 0 = converged. */
+size_t msp_eigh_workspace(int n, int batch, int method);
+int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
+             int method, void *d_work, size_t work_bytes, void *stream);
+
+/* Y[b] = (v[b].S_mu (x) 1) U[b] with v = d_vec (batch x 3) and the muon
+ * position/dims of d_systems[d_sys_id[b]].  pauli = 1 uses sigma = 2S. */
+int msp_apply_muon_op(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
+                      const double *d_vec, int pauli, const double *d_U, double *d_Y,
+                      void *stream);
+
+/* Y[b] = (F_0 (x) F_1 (x) ... (x) F_{k-1}) U[b] with per-instance complex
+ * factors d_F (batch x sum_k dims[k]^2, row-major per factor); d_tmp is
+ * scratch of the size of d_Y. */
+int msp_apply_kron(int n, int batch, int n_spins, const int32_t *h_dims, const double *d_F,
+                   const double *d_U, double *d_Y, double *d_tmp, void *stream);
+
+/* C[b] = A[b]^H B[b] (complex n x n). */
+int msp_gemm_cn(int n, int batch, const double *d_A, const double *d_B, double *d_C,
+                void *stream);
+
+/* Coefficient matrix of the time sum.
+ * real_mode = 0: C_ij = rho'_ij O'_ji (complex, d_rho required).
+ * real_mode = 1: C_ij = alpha |O'_ij|^2 + beta delta_ij Re O'_ii (real). */
+int msp_form_c(int n, int batch, const double *d_rho, const double *d_obs, int real_mode,
+               double alpha, double beta, double *d_C, void *stream);
+
+/* P_b(t) = sum_ij C_ij exp(-2 pi i (E_i - E_j) t) for each time.
+ * out_complex = 0: d_out[b*T + t] = w_b Re P_b(t) (w = 1 if d_w == NULL);
+ * out_complex = 1: d_out is complex, unweighted. */
+int msp_timesum(int n, int batch, int ntimes, const double *d_E, const double *d_C,
+                int real_mode, const double *d_times, const double *d_w, int out_complex,
+                double *d_out, void *stream);
+
+/* I_b = sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)) * out_scale.
+ * out_complex = 0: d_out[b] = w_b Re I_b; 1: complex, unweighted. */
+int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode,
+                 double tau, double out_scale, const double *d_w, int out_complex,
+                 double *d_out, void *stream);
+
+/* results[s][t] += rows[i][t] * scale for i in d_idx[d_ptr[s] .. d_ptr[s+1]),
+ * in that order (deterministic powder accumulation). */
+int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
+                   const double *d_rows, double scale, double *d_results, void *stream);
+
+/* Kernel timing (no reference counterpart; instrumentation for benchmarks):
+ * while enabled every kernel launch is bracketed by two hipEvents on its
+ * stream.  msp_profile_report waits for the recorded events, writes lines
+ * "kernel_name launches total_ms" into h_buf and clears the record. */
+int msp_profile_enable(int on);
+int msp_profile_report(char *h_buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+// Host build of the device eigensolver team functions (one thread, same
+// phase schedule), exported for CPU tests of the algorithms the HIP kernels
+// run.  Not part of the product library.
+#define MSP_HOST_BUILD 1
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "msp_jacobi.h"
+#include "msp_tridiag.h"
+
+extern "C" {
+
+// H: n x n complex row-major (copied); E ascending; U columns = eigenvectors.
+int hc_jacobi(int n, const double *H, double *E, double *U, int max_sweeps)
+{
+    std::vector<cdbl> A((size_t)n * n), V((size_t)n * n);
+    memcpy(A.data(), H, sizeof(cdbl) * n * n);
+    std::vector<JacRot> rot(n / 2 + 1);
+    double red[1];
+    int flag = 0;
+    int sweeps = jacobi_team(n, A.data(), n, V.data(), n, rot.data(), red, &flag, max_sweeps);
+    for (int i = 0; i < n; ++i) {
+        int r = jac_rank(n, A.data(), n, i);
+        E[r] = A[(size_t)i * n + i].re;
+        for (int row = 0; row < n; ++row) {
+            U[2 * ((size_t)row * n + r)] = V[(size_t)row * n + i].re;
+            U[2 * ((size_t)row * n + r) + 1] = V[(size_t)row * n + i].im;
+        }
+    }
+    return sweeps;
+}
+
+int hc_tridiag(int n, double *A, double *d, double *e, double *tau)
+{
+    std::vector<cdbl> vb(n), wb(n);
+    double red[1];
+    tridiag_team(n, (cdbl *)A, d, e, (cdbl *)tau, vb.data(), wb.data(), red);
+    return 0;
+}
+
+struct DcAlloc {
+    std::vector<double> dbl;
+    std::vector<int> in;
+    DcWork W;
+    DcAlloc(int n) : dbl((size_t)3 * n * n + 12 * (size_t)n), in(16 * (size_t)n)
+    {
+        double *p = dbl.data();
+        W.Q0 = p; p += (size_t)n * n;
+        W.Q1 = p; p += (size_t)n * n;
+        W.S = p; p += (size_t)n * n;
+        W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
+        W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
+        W.rc = p; p += n; W.rs = p; p += n; W.item_v = p; p += n; W.outv = p; p += n;
+        int *q = in.data();
+        W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
+        W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
+        W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n;
+    }
+};
+
+// d: in diagonal, out ascending eigenvalues; Z: n x n real eigenvectors (columns).
+int hc_stedc(int n, double *d, const double *e, double *Z)
+{
+    DcAlloc al(n);
+    double red[1];
+    double *Q = stedc_team(n, d, e, al.W, red);
+    memcpy(Z, Q, sizeof(double) * n * n);
+    return 0;
+}
+
+// Full path: tridiagonalise, divide and conquer, back-transform.
+int hc_eigh_dc(int n, const double *H, double *E, double *U, int cw)
+{
+    std::vector<cdbl> A((size_t)n * n);
+    memcpy(A.data(), H, sizeof(cdbl) * n * n);
+    std::vector<double> e(n);
+    std::vector<cdbl> tau(n), vb(n), wb(n), Uc((size_t)n * cw), ws(cw);
+    double red[1];
+    tridiag_team(n, A.data(), E, e.data(), tau.data(), vb.data(), wb.data(), red);
+    DcAlloc al(n);
+    double *Q = stedc_team(n, E, e.data(), al.W, red);
+    backtransform_team(n, A.data(), tau.data(), Q, (cdbl *)U, Uc.data(), cw, ws.data());
+    return 0;
+}
+}

@@ -1,0 +1,241 @@
+// extern "C" entry points of libmuspin_hip.so (declared in include/muspin_hip.h):
+// argument validation, error reporting and kernel dispatch.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/muspin_hip.h"
+#include "msp_kernels.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_status(hipError_t e, const char *where)
+{
+    if (e == hipSuccess) return MSP_OK;
+    return fail(MSP_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+constexpr int kMaxDim = 2048;
+constexpr int kJacobiMax = 64;
+
+}  // namespace
+
+extern "C" {
+
+int msp_version(void) { return 1; }
+
+int msp_last_error(char *h_buf, size_t len)
+{
+    if (!h_buf || len == 0) return MSP_ERR_VALUE;
+    strncpy(h_buf, g_err, len - 1);
+    h_buf[len - 1] = '\0';
+    return MSP_OK;
+}
+
+int msp_init(void)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_status(e, "msp_init");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_status(e, "msp_init");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(MSP_ERR_UNSUPPORTED, "msp_init: device is %s, library is built for gfx950",
+                    prop.gcnArchName);
+    return MSP_OK;
+}
+
+int msp_rotate_fields(int batch, const double *d_q, const double *d_B, const double *d_Bint,
+                      const double *d_p, double *d_Br, double *d_pr, void *stream)
+{
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_rotate_fields: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_q || !d_B || !d_Bint || !d_p || !d_Br || !d_pr)
+        return fail(MSP_ERR_VALUE, "msp_rotate_fields: null pointer");
+    return hip_status(launch_rotate(batch, d_q, d_B, d_Bint, d_p, d_Br, d_pr, S(stream)),
+                      "msp_rotate_fields");
+}
+
+int msp_build_h(int n, int batch, const MspSystem *d_systems, const MspTerm *d_terms,
+                const int32_t *d_sys_id, const double *d_Br, const double *d_base, int zeeman,
+                double *d_H, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_build_h: n = %d out of range", n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_build_h: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_systems || !d_H || (zeeman && !d_Br) || (!d_base && !d_terms))
+        return fail(MSP_ERR_VALUE, "msp_build_h: null pointer");
+    return hip_status(launch_build_h(n, batch, d_systems, d_terms, d_sys_id, d_Br, d_base, zeeman,
+                                     d_H, S(stream)),
+                      "msp_build_h");
+}
+
+size_t msp_eigh_workspace(int n, int batch, int method)
+{
+    if (n < 1 || batch < 1) return 0;
+    if (method == MSP_EIGH_AUTO) method = (n <= kJacobiMax) ? MSP_EIGH_JACOBI : MSP_EIGH_DC;
+    if (method == MSP_EIGH_JACOBI) return 0;
+    return dc_workspace_doubles(n) * sizeof(double) * (size_t)batch;
+}
+
+int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info, int method,
+             void *d_work, size_t work_bytes, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_eigh: n = %d out of range", n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_eigh: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_H || !d_E || !d_U || !d_info) return fail(MSP_ERR_VALUE, "msp_eigh: null pointer");
+    if (method == MSP_EIGH_AUTO) method = (n <= kJacobiMax) ? MSP_EIGH_JACOBI : MSP_EIGH_DC;
+    if (method == MSP_EIGH_JACOBI) {
+        if (n > kJacobiMax)
+            return fail(MSP_ERR_UNSUPPORTED, "msp_eigh: Jacobi path holds n <= %d in LDS (n = %d)",
+                        kJacobiMax, n);
+        return hip_status(launch_eigh_jacobi(n, batch, d_H, d_E, d_U, d_info, 40, S(stream)),
+                          "msp_eigh(jacobi)");
+    }
+    if (method != MSP_EIGH_DC) return fail(MSP_ERR_VALUE, "msp_eigh: unknown method %d", method);
+    size_t need = msp_eigh_workspace(n, batch, MSP_EIGH_DC);
+    if (!d_work || work_bytes < need)
+        return fail(MSP_ERR_VALUE, "msp_eigh: workspace of %zu bytes needed, %zu given", need,
+                    work_bytes);
+    return hip_status(launch_eigh_dc(n, batch, d_H, d_E, d_U, d_info, (double *)d_work, S(stream)),
+                      "msp_eigh(dc)");
+}
+
+int msp_apply_muon_op(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
+                      const double *d_vec, int pauli, const double *d_U, double *d_Y,
+                      void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_apply_muon_op: n out of range");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_apply_muon_op: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_systems || !d_vec || !d_U || !d_Y)
+        return fail(MSP_ERR_VALUE, "msp_apply_muon_op: null pointer");
+    return hip_status(launch_apply_muon(n, batch, d_systems, d_sys_id, d_vec, pauli, d_U, d_Y,
+                                        S(stream)),
+                      "msp_apply_muon_op");
+}
+
+int msp_apply_kron(int n, int batch, int n_spins, const int32_t *h_dims, const double *d_F,
+                   const double *d_U, double *d_Y, double *d_tmp, void *stream)
+{
+    if (n < 1 || n > kMaxDim || n_spins < 1 || n_spins > MSP_MAX_SPINS || !h_dims)
+        return fail(MSP_ERR_VALUE, "msp_apply_kron: bad dimensions");
+    long prod = 1, fstride = 0;
+    for (int k = 0; k < n_spins; ++k) {
+        if (h_dims[k] < 1) return fail(MSP_ERR_VALUE, "msp_apply_kron: bad spin dimension");
+        prod *= h_dims[k];
+        fstride += 2L * h_dims[k] * h_dims[k];
+    }
+    if (prod != n) return fail(MSP_ERR_VALUE, "msp_apply_kron: prod(dims) = %ld != n = %d", prod, n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_apply_kron: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_F || !d_U || !d_Y || !d_tmp) return fail(MSP_ERR_VALUE, "msp_apply_kron: null pointer");
+    // ping-pong so that the last product lands in d_Y
+    const double *src = d_U;
+    double *bufs[2] = {d_Y, d_tmp};
+    int dst = (n_spins % 2 == 1) ? 0 : 1;
+    long foff = 0;
+    int stride = n;
+    for (int k = 0; k < n_spins; ++k) {
+        stride /= h_dims[k];
+        hipError_t e = launch_mode_product(n, batch, h_dims[k], stride, (int)(foff / 2),
+                                           (int)(fstride / 2), d_F, src, bufs[dst], S(stream));
+        if (e != hipSuccess) return hip_status(e, "msp_apply_kron");
+        foff += 2L * h_dims[k] * h_dims[k];
+        src = bufs[dst];
+        dst ^= 1;
+    }
+    return MSP_OK;
+}
+
+int msp_gemm_cn(int n, int batch, const double *d_A, const double *d_B, double *d_C, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_gemm_cn: n out of range");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_gemm_cn: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_A || !d_B || !d_C) return fail(MSP_ERR_VALUE, "msp_gemm_cn: null pointer");
+    return hip_status(launch_gemm_cn(n, batch, d_A, d_B, d_C, S(stream)), "msp_gemm_cn");
+}
+
+int msp_form_c(int n, int batch, const double *d_rho, const double *d_obs, int real_mode,
+               double alpha, double beta, double *d_C, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_form_c: n out of range");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_form_c: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_obs || !d_C || (!real_mode && !d_rho)) return fail(MSP_ERR_VALUE, "msp_form_c: null pointer");
+    return hip_status(launch_form_c(n, batch, d_rho, d_obs, real_mode, alpha, beta, d_C, S(stream)),
+                      "msp_form_c");
+}
+
+int msp_timesum(int n, int batch, int ntimes, const double *d_E, const double *d_C, int real_mode,
+                const double *d_times, const double *d_w, int out_complex, double *d_out,
+                void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_timesum: n out of range");
+    if (batch < 0 || ntimes < 0) return fail(MSP_ERR_VALUE, "msp_timesum: negative size");
+    if (batch == 0 || ntimes == 0) return MSP_OK;
+    if (!d_E || !d_C || !d_times || !d_out) return fail(MSP_ERR_VALUE, "msp_timesum: null pointer");
+    if ((size_t)ntimes / 64 + 1 > 2147483647u) return fail(MSP_ERR_VALUE, "msp_timesum: too many times");
+    return hip_status(launch_timesum(n, batch, ntimes, d_E, d_C, real_mode, d_times, d_w,
+                                     out_complex, d_out, S(stream)),
+                      "msp_timesum");
+}
+
+int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode, double tau,
+                 double out_scale, const double *d_w, int out_complex, double *d_out, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_integral: n out of range");
+    if (!(tau > 0.0)) return fail(MSP_ERR_VALUE, "msp_integral: tau must be > 0");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_integral: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_E || !d_C || !d_out) return fail(MSP_ERR_VALUE, "msp_integral: null pointer");
+    return hip_status(launch_integral(n, batch, d_E, d_C, real_mode, tau, out_scale, d_w,
+                                      out_complex, d_out, S(stream)),
+                      "msp_integral");
+}
+
+int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
+                   const double *d_rows, double scale, double *d_results, void *stream)
+{
+    if (nslots < 0 || width < 0) return fail(MSP_ERR_VALUE, "msp_accumulate: negative size");
+    if (nslots == 0 || width == 0) return MSP_OK;
+    if (!d_ptr || !d_idx || !d_rows || !d_results)
+        return fail(MSP_ERR_VALUE, "msp_accumulate: null pointer");
+    return hip_status(launch_accumulate(nslots, width, d_ptr, d_idx, d_rows, scale, d_results,
+                                        S(stream)),
+                      "msp_accumulate");
+}
+
+int msp_profile_enable(int on)
+{
+    prof_enable(on);
+    return MSP_OK;
+}
+
+int msp_profile_report(char *h_buf, size_t len)
+{
+    if (!h_buf || len == 0) return fail(MSP_ERR_VALUE, "msp_profile_report: empty buffer");
+    std::string r = prof_report();
+    if (r.size() + 1 > len) return fail(MSP_ERR_VALUE, "msp_profile_report: buffer too small");
+    memcpy(h_buf, r.c_str(), r.size() + 1);
+    return MSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+// Host-side launchers of the kernels in msp_kernels.hip (no argument checks;
+// msp_capi.hip validates before calling).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+struct MspSystem;
+struct MspTerm;
+
+size_t jacobi_lds_bytes(int n);
+size_t dc_workspace_doubles(int n);
+int dc_chunk_width(int n);
+
+hipError_t launch_rotate(int batch, const double *q, const double *B, const double *Bint,
+                         const double *p, double *Br, double *pr, hipStream_t s);
+hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspTerm *terms,
+                          const int32_t *sys_id, const double *Br, const double *base, int zeeman,
+                          double *H, hipStream_t s);
+hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, double *U,
+                              int32_t *info, int max_sweeps, hipStream_t s);
+hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
+                          double *work, hipStream_t s);
+hipError_t launch_apply_muon(int n, int batch, const MspSystem *systems, const int32_t *sys_id,
+                             const double *vec, int pauli, const double *U, double *Y,
+                             hipStream_t s);
+hipError_t launch_mode_product(int n, int batch, int dk, int stride, int foff, int fstride,
+                               const double *F, const double *X, double *Y, hipStream_t s);
+hipError_t launch_gemm_cn(int n, int batch, const double *A, const double *B, double *C,
+                          hipStream_t s);
+hipError_t launch_form_c(int n, int batch, const double *rho, const double *obs, int real_mode,
+                         double alpha, double beta, double *C, hipStream_t s);
+hipError_t launch_timesum(int n, int batch, int ntimes, const double *E, const double *C,
+                          int real_mode, const double *times, const double *w, int out_complex,
+                          double *out, hipStream_t s);
+hipError_t launch_integral(int n, int batch, const double *E, const double *C, int real_mode,
+                           double tau, double out_scale, const double *w, int out_complex,
+                           double *out, hipStream_t s);
+hipError_t launch_accumulate(int nslots, int width, const int32_t *ptr, const int32_t *idx,
+                             const double *rows, double scale, double *results, hipStream_t s);
+
+void prof_enable(int on);
+std::string prof_report();

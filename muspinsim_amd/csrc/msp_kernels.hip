@@ -1,0 +1,745 @@
+// HIP kernels of the spin-dynamics path, gfx950.  Launch wrappers and
+// argument checks live in msp_capi.hip; the eigensolver algorithms are the
+// team functions of msp_jacobi.h / msp_tridiag.h.
+#include <hip/hip_runtime.h>
+
+#include "../../include/muspin_hip.h"
+#include "msp_jacobi.h"
+#include "msp_kernels.h"
+#include "msp_tridiag.h"
+
+// <r|S^ax|c> for one spin of multiplicity d (m_k = I - k), ax = 0,1,2 = x,y,z.
+// Raising elements <k|S+|k+1> = sqrt((k+1)(2I - k)) (muspinsim/spinop.py:18-41).
+__device__ inline cdbl spin_el(int d, int ax, int r, int c)
+{
+    if (ax == 2) return (r == c) ? cmk(0.5 * (d - 1) - r, 0.0) : cmk(0.0, 0.0);
+    if (r + 1 == c) {
+        double sp = sqrt((double)(r + 1) * (double)(d - 1 - r));
+        return ax == 0 ? cmk(0.5 * sp, 0.0) : cmk(0.0, -0.5 * sp);
+    }
+    if (c + 1 == r) {
+        double sp = sqrt((double)(c + 1) * (double)(d - 1 - c));
+        return ax == 0 ? cmk(0.5 * sp, 0.0) : cmk(0.0, 0.5 * sp);
+    }
+    return cmk(0.0, 0.0);
+}
+
+// <r|S^a S^b|c> (same-spin product, quadrupolar terms)
+__device__ inline cdbl spin_el2(int d, int a, int b, int r, int c)
+{
+    cdbl acc = cmk(0.0, 0.0);
+    for (int m = r - 1; m <= r + 1; ++m) {
+        if (m < 0 || m >= d) continue;
+        acc = cadd(acc, cmul(spin_el(d, a, r, m), spin_el(d, b, m, c)));
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_rotate(int batch, const double *q, const double *B, const double *Bint,
+                         const double *p, double *Br, double *pr)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const double w = q[4 * b], x = q[4 * b + 1], y = q[4 * b + 2], z = q[4 * b + 3];
+    // v' = q v q*: v + 2w (u x v) + 2 u x (u x v), u = (x, y, z)
+    for (int which = 0; which < 2; ++which) {
+        const double *v = which ? p + 3 * b : B + 3 * b;
+        double cx = y * v[2] - z * v[1], cy = z * v[0] - x * v[2], cz = x * v[1] - y * v[0];
+        double ddx = y * cz - z * cy, ddy = z * cx - x * cz, ddz = x * cy - y * cx;
+        double o0 = v[0] + 2.0 * (w * cx + ddx);
+        double o1 = v[1] + 2.0 * (w * cy + ddy);
+        double o2 = v[2] + 2.0 * (w * cz + ddz);
+        if (which) {
+            pr[3 * b] = o0; pr[3 * b + 1] = o1; pr[3 * b + 2] = o2;
+        } else {
+            Br[3 * b] = o0 + Bint[3 * b];
+            Br[3 * b + 1] = o1 + Bint[3 * b + 1];
+            Br[3 * b + 2] = o2 + Bint[3 * b + 2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// H[b][a][c]: terms (or a precomputed base) + Zeeman, one thread per element.
+__global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
+                          const int32_t *sys_id, const double *Br, const double *base,
+                          int zeeman, double *H)
+{
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int s = sys_id ? sys_id[b] : 0;
+    const MspSystem &S = systems[s];
+    const int ar = idx / n, ac = idx % n;
+    int dr[MSP_MAX_SPINS], dc[MSP_MAX_SPINS];
+    int ndiff = 0, p0 = -1, p1 = -1;
+    {
+        int x = ar, y = ac;
+        for (int k = S.n_spins - 1; k >= 0; --k) {
+            int d = S.dims[k];
+            dr[k] = x % d;
+            dc[k] = y % d;
+            x /= d;
+            y /= d;
+            if (dr[k] != dc[k]) {
+                if (ndiff == 0) p0 = k; else if (ndiff == 1) p1 = k;
+                ++ndiff;
+            }
+        }
+    }
+    cdbl acc = cmk(0.0, 0.0);
+    if (base) {
+        const cdbl *bm = (const cdbl *)base + (size_t)s * n * n;
+        acc = bm[idx];
+    }
+    if (ndiff <= 2) {
+        if (!base) {
+            for (int t = 0; t < S.n_terms; ++t) {
+                const MspTerm &T = terms[S.term_offset + t];
+                if (T.kind == 0) {
+                    if (ndiff == 0) acc.re += T.t[0];
+                } else if (T.kind == 1) {
+                    int i = T.i;
+                    if (ndiff == 0 || (ndiff == 1 && p0 == i)) {
+                        for (int a = 0; a < 3; ++a)
+                            acc = cadd(acc, cscale(spin_el(S.dims[i], a, dr[i], dc[i]), T.t[a]));
+                    }
+                } else {
+                    int i = T.i, j = T.j;
+                    if (i != j) {
+                        bool ok = (ndiff == 0) || (ndiff == 1 && (p0 == i || p0 == j)) ||
+                                  (ndiff == 2 && ((p0 == i && p1 == j) || (p0 == j && p1 == i)));
+                        if (ok) {
+                            cdbl ei[3], ej[3];
+                            for (int a = 0; a < 3; ++a) {
+                                ei[a] = spin_el(S.dims[i], a, dr[i], dc[i]);
+                                ej[a] = spin_el(S.dims[j], a, dr[j], dc[j]);
+                            }
+                            for (int a = 0; a < 3; ++a)
+                                for (int c = 0; c < 3; ++c) {
+                                    double m = T.t[3 * a + c];
+                                    if (m != 0.0) acc = cadd(acc, cscale(cmul(ei[a], ej[c]), m));
+                                }
+                        }
+                    } else if (ndiff == 0 || (ndiff == 1 && p0 == i)) {
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) {
+                                double m = T.t[3 * a + c];
+                                if (m != 0.0)
+                                    acc = cadd(acc, cscale(spin_el2(S.dims[i], a, c, dr[i], dc[i]), m));
+                            }
+                    }
+                }
+            }
+        }
+        if (zeeman && ndiff <= 1) {
+            const double *B = Br + 3 * b;
+            for (int i = 0; i < S.n_spins; ++i) {
+                if (ndiff == 1 && p0 != i) continue;
+                double g = S.gammas[i];
+                for (int a = 0; a < 3; ++a)
+                    acc = cadd(acc, cscale(spin_el(S.dims[i], a, dr[i], dc[i]), g * B[a]));
+            }
+        }
+    }
+    ((cdbl *)H)[(size_t)b * n * n + idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi eigensolver: one workgroup per matrix, A and V in LDS.
+__global__ void k_eigh_jacobi(int n, const double *H, double *E, double *U, int32_t *info,
+                              int max_sweeps)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.x;
+    const int ld = n + 1;
+    const int m = n + (n & 1);
+    cdbl *A = (cdbl *)smem;
+    cdbl *V = A + (size_t)n * ld;
+    JacRot *rot = (JacRot *)(V + (size_t)n * ld);
+    double *red = (double *)(rot + m / 2);
+    int *flag = (int *)(red + 16);
+    int *rank = flag + 1;
+    const cdbl *Hb = (const cdbl *)H + (size_t)b * n * n;
+    MSP_PFOR(idx, n * n) A[(idx / n) * ld + idx % n] = Hb[idx];
+    __syncthreads();
+    int sweeps = jacobi_team(n, A, ld, V, ld, rot, red, flag, max_sweeps);
+    __syncthreads();
+    MSP_PFOR(i, n)
+    {
+        int r = jac_rank(n, A, ld, i);
+        rank[i] = r;
+        E[(size_t)b * n + r] = A[i * ld + i].re;
+    }
+    __syncthreads();
+    cdbl *Ub = (cdbl *)U + (size_t)b * n * n;
+    MSP_PFOR(idx, n * n)
+    {
+        int row = idx / n, j = idx % n;
+        Ub[(size_t)row * n + rank[j]] = V[row * ld + j];
+    }
+    if (threadIdx.x == 0) info[b] = (sweeps >= max_sweeps) ? 1 : 0;
+}
+
+size_t jacobi_lds_bytes(int n)
+{
+    const int ld = n + 1, m = n + (n & 1);
+    return 2 * sizeof(cdbl) * (size_t)n * ld + sizeof(JacRot) * (m / 2) + 16 * sizeof(double) +
+           sizeof(int) * (1 + n) + 64;
+}
+
+// ---------------------------------------------------------------------------
+// Divide-and-conquer path, three kernels per batch.
+__global__ void k_tridiag(int n, double *H, double *E, double *work)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.x;
+    cdbl *vb = (cdbl *)smem;
+    cdbl *wb = vb + n;
+    double *red = (double *)(wb + n);
+    double *e;
+    cdbl *tau;
+    dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
+    tridiag_team(n, (cdbl *)H + (size_t)b * n * n, E + (size_t)b * n, e, tau, vb, wb, red);
+}
+
+__global__ void k_stedc(int n, double *E, double *work, int32_t *info)
+{
+    __shared__ double red[16];
+    const int b = blockIdx.x;
+    double *e;
+    cdbl *tau;
+    DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
+    stedc_team(n, E + (size_t)b * n, e, W, red);
+    if (threadIdx.x == 0) info[b] = 0;
+}
+
+__global__ void k_backtransform(int n, const double *H, double *work, double *U, int cw)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.y;
+    const int c0 = blockIdx.x * cw;
+    cdbl *Uc = (cdbl *)smem;
+    cdbl *ws = Uc + (size_t)n * cw;
+    double *e;
+    cdbl *tau;
+    DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
+    backtransform_chunk_team(n, (const cdbl *)H + (size_t)b * n * n, tau, dc_result(n, W),
+                             (cdbl *)U + (size_t)b * n * n, Uc, cw, c0, ws);
+}
+
+// ---------------------------------------------------------------------------
+// Y = (v.S_mu (x) 1) U (pauli: sigma = 2S).
+__global__ void k_apply_muon(int n, const MspSystem *systems, const int32_t *sys_id,
+                             const double *vec, int pauli, const double *U, double *Y)
+{
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const MspSystem &S = systems[sys_id ? sys_id[b] : 0];
+    const int mu = S.muon_index;
+    int stride = 1;
+    for (int k = S.n_spins - 1; k > mu; --k) stride *= S.dims[k];
+    const int dm = S.dims[mu];
+    const int row = idx / n, col = idx % n;
+    const int am = (row / stride) % dm;
+    const double *v = vec + 3 * b;
+    const double f = pauli ? 2.0 : 1.0;
+    const cdbl *Ub = (const cdbl *)U + (size_t)b * n * n;
+    cdbl acc = cmk(0.0, 0.0);
+    for (int mp = 0; mp < dm; ++mp) {
+        cdbl mel = cmk(0.0, 0.0);
+        for (int a = 0; a < 3; ++a) mel = cadd(mel, cscale(spin_el(dm, a, am, mp), f * v[a]));
+        if (mel.re == 0.0 && mel.im == 0.0) continue;
+        int r2 = row + (mp - am) * stride;
+        acc = cadd(acc, cmul(mel, Ub[(size_t)r2 * n + col]));
+    }
+    ((cdbl *)Y)[(size_t)b * n * n + idx] = acc;
+}
+
+// One mode product: Y[a][c] = sum_m F[a_k][m] X[a | digit k = m][c].
+__global__ void k_mode_product(int n, int dk, int stride, int foff, int fstride, const double *F,
+                               const double *X, double *Y)
+{
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int row = idx / n, col = idx % n;
+    const int ak = (row / stride) % dk;
+    const cdbl *Fb = (const cdbl *)F + (size_t)b * fstride + foff;
+    const cdbl *Xb = (const cdbl *)X + (size_t)b * n * n;
+    cdbl acc = cmk(0.0, 0.0);
+    for (int mp = 0; mp < dk; ++mp) {
+        int r2 = row + (mp - ak) * stride;
+        cfma(acc, Fb[ak * dk + mp], Xb[(size_t)r2 * n + col]);
+    }
+    ((cdbl *)Y)[(size_t)b * n * n + idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// C = A^H B, 32 x 32 output tile per workgroup of 256 threads, k-chunks of 16.
+__global__ void __launch_bounds__(256) k_gemm_cn(int n, const double *A, const double *B, double *C)
+{
+    __shared__ cdbl As[16][33];
+    __shared__ cdbl Bs[16][33];
+    const int b = blockIdx.z;
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    const cdbl *Ab = (const cdbl *)A + (size_t)b * n * n;
+    const cdbl *Bb = (const cdbl *)B + (size_t)b * n * n;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    cdbl acc[2][2];
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) acc[r][c] = cmk(0.0, 0.0);
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        for (int l = threadIdx.x; l < 16 * 32; l += 256) {
+            int kk = l / 32, cc = l % 32;
+            int k = k0 + kk;
+            As[kk][cc] = (k < n && i0 + cc < n) ? Ab[(size_t)k * n + i0 + cc] : cmk(0.0, 0.0);
+            Bs[kk][cc] = (k < n && j0 + cc < n) ? Bb[(size_t)k * n + j0 + cc] : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            cdbl a0 = As[kk][ty * 2], a1 = As[kk][ty * 2 + 1];
+            cdbl b0 = Bs[kk][tx * 2], b1 = Bs[kk][tx * 2 + 1];
+            cfmac(acc[0][0], a0, b0);
+            cfmac(acc[0][1], a0, b1);
+            cfmac(acc[1][0], a1, b0);
+            cfmac(acc[1][1], a1, b1);
+        }
+        __syncthreads();
+    }
+    cdbl *Cb = (cdbl *)C + (size_t)b * n * n;
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) {
+            int i = i0 + ty * 2 + r, j = j0 + tx * 2 + c;
+            if (i < n && j < n) Cb[(size_t)i * n + j] = acc[r][c];
+        }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_form_c(int n, const double *rho, const double *obs, int real_mode,
+                         double alpha, double beta, double *C)
+{
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int i = idx / n, j = idx % n;
+    const cdbl *O = (const cdbl *)obs + (size_t)b * n * n;
+    if (real_mode) {
+        double v = alpha * cabs2(O[idx]);
+        if (i == j) v += beta * O[idx].re;
+        C[(size_t)b * n * n + idx] = v;
+    } else {
+        const cdbl *R = (const cdbl *)rho + (size_t)b * n * n;
+        ((cdbl *)C)[(size_t)b * n * n + idx] = cmul(R[idx], O[(size_t)j * n + i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Time sum P(t) = sum_i conj(F_i(t)) sum_j C_ij F_j(t), F_j(t) = exp(i phi_j(t)),
+// phi_j = 2 pi (E_j - E_c) t with E_c the spectrum midpoint (phases formed in
+// FP64 from centred energies).  Workgroup = (instance, 64 times); 256 threads
+// = 16 row groups x 16 time groups, 4 x 4 complex accumulators each.
+template <bool REAL>
+__global__ void __launch_bounds__(256) k_timesum(int n, int ntimes, const double *E,
+                                                 const double *C, const double *times,
+                                                 const double *w, int out_complex,
+                                                 double *out)
+{
+    constexpr int TI = 64, TT = 64, TJ = 16;
+    __shared__ double Cs_re[TI][TJ + 1];
+    __shared__ double Cs_im[REAL ? 1 : TI][TJ + 1];
+    __shared__ cdbl Ph[TJ][TT + 1];
+    __shared__ cdbl red[16][TT + 1];
+    const int b = blockIdx.y;
+    const int t0 = blockIdx.x * TT;
+    const int tg = threadIdx.x % 16, ig = threadIdx.x / 16;
+    const double *Eb = E + (size_t)b * n;
+    const double Ec = 0.5 * (Eb[0] + Eb[n - 1]);
+    const double twopi = 6.283185307179586;
+    double tloc[4];
+    for (int s = 0; s < 4; ++s) {
+        int t = t0 + tg * 4 + s;
+        tloc[s] = (t < ntimes) ? times[t] : 0.0;
+    }
+    cdbl Pacc[4];
+    for (int s = 0; s < 4; ++s) Pacc[s] = cmk(0.0, 0.0);
+    for (int i0 = 0; i0 < n; i0 += TI) {
+        cdbl acc[4][4];
+        for (int r = 0; r < 4; ++r)
+            for (int s = 0; s < 4; ++s) acc[r][s] = cmk(0.0, 0.0);
+        for (int j0 = 0; j0 < n; j0 += TJ) {
+            for (int l = threadIdx.x; l < TI * TJ; l += 256) {
+                int r = l / TJ, c = l % TJ;
+                int i = i0 + r, j = j0 + c;
+                bool ok = (i < n && j < n);
+                if (REAL) {
+                    Cs_re[r][c] = ok ? C[(size_t)b * n * n + (size_t)i * n + j] : 0.0;
+                } else {
+                    cdbl v = ok ? ((const cdbl *)C)[(size_t)b * n * n + (size_t)i * n + j] : cmk(0.0, 0.0);
+                    Cs_re[r][c] = v.re;
+                    Cs_im[r][c] = v.im;
+                }
+            }
+            for (int l = threadIdx.x; l < TJ * TT; l += 256) {
+                int jj = l / TT, tt = l % TT;
+                int j = j0 + jj, t = t0 + tt;
+                cdbl ph = cmk(0.0, 0.0);
+                if (j < n && t < ntimes) {
+                    double phi = twopi * (Eb[j] - Ec) * times[t];
+                    double sn, cs;
+                    sincos(phi, &sn, &cs);
+                    ph = cmk(cs, sn);
+                }
+                Ph[jj][tt] = ph;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int jj = 0; jj < TJ; ++jj) {
+                cdbl p[4];
+                for (int s = 0; s < 4; ++s) p[s] = Ph[jj][tg * 4 + s];
+                for (int r = 0; r < 4; ++r) {
+                    double cr = Cs_re[ig * 4 + r][jj];
+                    if (REAL) {
+                        for (int s = 0; s < 4; ++s) {
+                            acc[r][s].re = fma(cr, p[s].re, acc[r][s].re);
+                            acc[r][s].im = fma(cr, p[s].im, acc[r][s].im);
+                        }
+                    } else {
+                        cdbl c = cmk(cr, Cs_im[ig * 4 + r][jj]);
+                        for (int s = 0; s < 4; ++s) cfma(acc[r][s], c, p[s]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // contract the rows with conj(F_i(t))
+        for (int s = 0; s < 4; ++s) {
+            cdbl part = cmk(0.0, 0.0);
+            for (int r = 0; r < 4; ++r) {
+                int i = i0 + ig * 4 + r;
+                if (i >= n) continue;
+                double phi = twopi * (Eb[i] - Ec) * tloc[s];
+                double sn, cs;
+                sincos(phi, &sn, &cs);
+                cfmac(part, cmk(cs, sn), acc[r][s]);
+            }
+            Pacc[s] = cadd(Pacc[s], part);
+        }
+    }
+    for (int s = 0; s < 4; ++s) red[ig][tg * 4 + s] = Pacc[s];
+    __syncthreads();
+    if (threadIdx.x < TT) {
+        int tt = threadIdx.x, t = t0 + tt;
+        cdbl P = cmk(0.0, 0.0);
+        for (int g = 0; g < 16; ++g) P = cadd(P, red[g][tt]);
+        if (t < ntimes) {
+            if (out_complex) {
+                out[2 * ((size_t)b * ntimes + t)] = P.re;
+                out[2 * ((size_t)b * ntimes + t) + 1] = P.im;
+            } else {
+                double wb = w ? w[b] : 1.0;
+                out[(size_t)b * ntimes + t] = wb * P.re;
+            }
+        }
+    }
+}
+
+template __global__ void k_timesum<true>(int, int, const double *, const double *, const double *,
+                                         const double *, int, double *);
+template __global__ void k_timesum<false>(int, int, const double *, const double *, const double *,
+                                          const double *, int, double *);
+
+// ---------------------------------------------------------------------------
+// I = sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)), one workgroup per instance.
+__global__ void __launch_bounds__(256) k_integral(int n, const double *E, const double *C,
+                                                  int real_mode, double tau, double out_scale,
+                                                  const double *w, int out_complex, double *out)
+{
+    __shared__ double red[16];
+    const int b = blockIdx.x;
+    const double *Eb = E + (size_t)b * n;
+    const double a = 1.0 / tau;
+    const double twopi = 6.283185307179586;
+    double sre = 0.0, sim = 0.0;
+    MSP_PFOR(idx, n * n)
+    {
+        int i = idx / n, j = idx % n;
+        double bb = twopi * (Eb[i] - Eb[j]);
+        double den = a * a + bb * bb;
+        if (real_mode) {
+            double c = C[(size_t)b * n * n + idx];
+            sre += c * a / den;
+            sim -= c * bb / den;
+        } else {
+            cdbl c = ((const cdbl *)C)[(size_t)b * n * n + idx];
+            // c / (a + i bb) = c (a - i bb) / den
+            sre += (c.re * a + c.im * bb) / den;
+            sim += (c.im * a - c.re * bb) / den;
+        }
+    }
+    sre = team_sum(sre, red);
+    sim = team_sum(sim, red);
+    if (threadIdx.x == 0) {
+        if (out_complex) {
+            out[2 * b] = sre * out_scale;
+            out[2 * b + 1] = sim * out_scale;
+        } else {
+            out[b] = (w ? w[b] : 1.0) * sre * out_scale;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_accumulate(int nslots, int width, const int32_t *ptr, const int32_t *idx,
+                             const double *rows, double scale, double *results)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots * width) return;
+    const int s = g / width, t = g % width;
+    double acc = results[g];
+    for (int k = ptr[s]; k < ptr[s + 1]; ++k) acc += rows[(size_t)idx[k] * width + t] * scale;
+    results[g] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing: when enabled, every launch below is bracketed by
+// two hipEvents on its stream; msp_profile_report sums them per kernel name.
+#include <string>
+#include <vector>
+
+namespace {
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t prof_event()
+{
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ProfScope {
+    const char *name;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(const char *n, hipStream_t st) : name(n), s(st)
+    {
+        if (g_prof && (a = prof_event())) hipEventRecord(a, s);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = prof_event();
+        if (!b) return;
+        hipEventRecord(b, s);
+        g_recs.push_back({name, a, b});
+    }
+};
+}  // namespace
+
+void prof_enable(int on) { g_prof = on != 0; }
+
+// "name calls total_ms\n" lines; synchronises on the recorded events and resets.
+std::string prof_report()
+{
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<long> cnt;
+    for (auto &r : g_recs) {
+        hipEventSynchronize(r.b);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        size_t k = 0;
+        for (; k < names.size(); ++k)
+            if (names[k] == r.name) break;
+        if (k == names.size()) {
+            names.push_back(r.name);
+            tot.push_back(0.0);
+            cnt.push_back(0);
+        }
+        tot[k] += ms;
+        cnt[k] += 1;
+        g_pool.push_back(r.a);
+        g_pool.push_back(r.b);
+    }
+    g_recs.clear();
+    std::string out;
+    char line[160];
+    for (size_t k = 0; k < names.size(); ++k) {
+        snprintf(line, sizeof(line), "%s %ld %.6f\n", names[k].c_str(), cnt[k], tot[k]);
+        out += line;
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+#define PROF(name) ProfScope prof_scope_(name, s)
+
+size_t dc_workspace_doubles(int n) { return dc_work_doubles(n); }
+
+// Back-transformation chunk width: as many columns as fit ~128 KB of LDS.
+int dc_chunk_width(int n)
+{
+    int cw = 64;
+    while (cw > 4 && (size_t)n * cw * sizeof(cdbl) > 128 * 1024) cw /= 2;
+    return cw;
+}
+
+hipError_t launch_rotate(int batch, const double *q, const double *B, const double *Bint,
+                         const double *p, double *Br, double *pr, hipStream_t s)
+{
+    { ProfScope ps_("k_rotate", s);
+    hipLaunchKernelGGL(k_rotate, dim3(cdiv(batch, 256)), dim3(256), 0, s, batch, q, B, Bint, p, Br, pr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspTerm *terms,
+                          const int32_t *sys_id, const double *Br, const double *base, int zeeman,
+                          double *H, hipStream_t s)
+{
+    { ProfScope ps_("k_build_h", s);
+    hipLaunchKernelGGL(k_build_h, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
+                       systems, terms, sys_id, Br, base, zeeman, H);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, double *U,
+                              int32_t *info, int max_sweeps, hipStream_t s)
+{
+    const int m = n + (n & 1);
+    int work = (m / 2) * (m / 2);
+    int nt = ((work + 63) / 64) * 64;
+    if (nt < 64) nt = 64;
+    if (nt > 1024) nt = 1024;
+    size_t lds = jacobi_lds_bytes(n);
+    hipError_t err = hipFuncSetAttribute((const void *)k_eigh_jacobi,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    { ProfScope ps_("k_eigh_jacobi", s);
+    hipLaunchKernelGGL(k_eigh_jacobi, dim3(batch), dim3(nt), lds, s, n, H, E, U, info, max_sweeps);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
+                          double *work, hipStream_t s)
+{
+    size_t lds1 = 2 * sizeof(cdbl) * (size_t)n + 16 * sizeof(double);
+    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    if (err != hipSuccess) return err;
+    { ProfScope ps_("k_tridiag", s);
+    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(256), lds1, s, n, H, E, work);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    { ProfScope ps_("k_stedc", s);
+    hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), 0, s, n, E, work, info);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int cw = dc_chunk_width(n);
+    size_t lds3 = sizeof(cdbl) * ((size_t)n * cw + cw);
+    err = hipFuncSetAttribute((const void *)k_backtransform,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);
+    if (err != hipSuccess) return err;
+    { ProfScope ps_("k_backtransform", s);
+    hipLaunchKernelGGL(k_backtransform, dim3(cdiv(n, cw), batch), dim3(256), lds3, s, n,
+                       (const double *)H, work, U, cw);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_muon(int n, int batch, const MspSystem *systems, const int32_t *sys_id,
+                             const double *vec, int pauli, const double *U, double *Y,
+                             hipStream_t s)
+{
+    { ProfScope ps_("k_apply_muon", s);
+    hipLaunchKernelGGL(k_apply_muon, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
+                       systems, sys_id, vec, pauli, U, Y);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mode_product(int n, int batch, int dk, int stride, int foff, int fstride,
+                               const double *F, const double *X, double *Y, hipStream_t s)
+{
+    { ProfScope ps_("k_mode_product", s);
+    hipLaunchKernelGGL(k_mode_product, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
+                       dk, stride, foff, fstride, F, X, Y);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_cn(int n, int batch, const double *A, const double *B, double *C,
+                          hipStream_t s)
+{
+    { ProfScope ps_("k_gemm_cn", s);
+    hipLaunchKernelGGL(k_gemm_cn, dim3(cdiv(n, 32), cdiv(n, 32), batch), dim3(256), 0, s, n, A, B, C);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_form_c(int n, int batch, const double *rho, const double *obs, int real_mode,
+                         double alpha, double beta, double *C, hipStream_t s)
+{
+    { ProfScope ps_("k_form_c", s);
+    hipLaunchKernelGGL(k_form_c, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n, rho,
+                       obs, real_mode, alpha, beta, C);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_timesum(int n, int batch, int ntimes, const double *E, const double *C,
+                          int real_mode, const double *times, const double *w, int out_complex,
+                          double *out, hipStream_t s)
+{
+    dim3 grid(cdiv(ntimes, 64), batch);
+    if (real_mode)
+        { ProfScope ps_("k_timesum", s);
+        hipLaunchKernelGGL(k_timesum<true>, grid, dim3(256), 0, s, n, ntimes, E, C, times, w,
+                           out_complex, out);
+        }
+    else
+        { ProfScope ps_("k_timesum", s);
+        hipLaunchKernelGGL(k_timesum<false>, grid, dim3(256), 0, s, n, ntimes, E, C, times, w,
+                           out_complex, out);
+        }
+    return hipGetLastError();
+}
+
+hipError_t launch_integral(int n, int batch, const double *E, const double *C, int real_mode,
+                           double tau, double out_scale, const double *w, int out_complex,
+                           double *out, hipStream_t s)
+{
+    { ProfScope ps_("k_integral", s);
+    hipLaunchKernelGGL(k_integral, dim3(batch), dim3(256), 0, s, n, E, C, real_mode, tau,
+                       out_scale, w, out_complex, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(int nslots, int width, const int32_t *ptr, const int32_t *idx,
+                             const double *rows, double scale, double *results, hipStream_t s)
+{
+    { ProfScope ps_("k_accumulate", s);
+    hipLaunchKernelGGL(k_accumulate, dim3(cdiv((size_t)nslots * width, 256)), dim3(256), 0, s,
+                       nslots, width, ptr, idx, rows, scale, results);
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,609 @@
+// Hermitian eigensolver for dimensions above the LDS-resident Jacobi:
+//   (1) Householder reduction A = Q T Q^H to a real symmetric tridiagonal T,
+//   (2) divide and conquer on T (Cuppen's split, Gu-Eisenstat eigenvectors),
+//   (3) back-transformation U = Q Z.
+// Replaces numpy.linalg.eigh (LAPACK zheevd) in Hermitian.diag
+// (muspinsim/spinop.py:69) with the same output contract: eigenvalues
+// ascending, orthonormal eigenvector columns.
+//
+// All three stages are team functions (see msp_team.h): one team owns one
+// matrix; the matrix and the divide-and-conquer workspaces live in global
+// memory, the back-transformation works on LDS-resident column chunks.
+#pragma once
+#include "msp_team.h"
+
+// ---------------------------------------------------------------------------
+// (1) Householder tridiagonalisation, full Hermitian storage (row-major,
+// row stride n).  Step k reflects column k below the diagonal:
+//   v = [1, x(1:)/(alpha - beta)], beta = -sign(Re alpha) ||x||, tau = (beta - alpha)/beta
+//   A22 <- H^H A22 H with H = I - tau v v^H, applied as the rank-2 update
+//   A22 -= v w^H + w v^H, p = tau A22 v, w = p - (tau/2)(p^H v) v.
+// Both triangles of A22 are kept, so p_i = sum_j conj(A22[j][i]) v_j reads
+// columns: consecutive threads touch consecutive addresses.
+// On exit d (n), e (n-1) hold T, tau (n-1) the reflector scalars and the
+// strictly-lower part of column k (rows k+2..n-1) the tail of v_k.
+// vbuf/wbuf: team-shared complex scratch of n entries each.
+MSP_HD void tridiag_team(int n, cdbl *A, double *d, double *e, cdbl *tau,
+                         cdbl *vbuf, cdbl *wbuf, double *red)
+{
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;          // size of the trailing block
+        cdbl *col = A + (size_t)(k + 1) * n + k;  // A[k+1][k], stride n
+        // ||x(1:)||^2
+        double part = 0.0;
+        MSP_PFOR(i, m - 1) part += cabs2(col[(size_t)(i + 1) * n]);
+        const double xn2 = team_sum(part, red);
+        const cdbl alpha = col[0];
+        double beta;
+        cdbl tk;
+        cdbl scale;
+        if (xn2 == 0.0 && alpha.im == 0.0) {
+            tk = cmk(0.0, 0.0);
+            beta = alpha.re;
+            scale = cmk(0.0, 0.0);
+        } else {
+            double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+            beta = alpha.re >= 0.0 ? -nrm : nrm;
+            tk = cmk((beta - alpha.re) / beta, -alpha.im / beta);
+            scale = cdivc(cmk(1.0, 0.0), csub(alpha, cmk(beta, 0.0)));
+        }
+        // v into vbuf and into the column (tail only)
+        MSP_PFOR(i, m)
+        {
+            cdbl vi;
+            if (i == 0) vi = cmk(1.0, 0.0);
+            else {
+                vi = cmul(col[(size_t)i * n], scale);
+                col[(size_t)i * n] = vi;
+            }
+            vbuf[i] = vi;
+        }
+        MSP_SYNC();
+        cdbl *A22 = A + (size_t)(k + 1) * n + (k + 1);
+        if (tk.re != 0.0 || tk.im != 0.0) {
+            // p = tau A22 v
+            MSP_PFOR(i, m)
+            {
+                cdbl acc = cmk(0.0, 0.0);
+                for (int j = 0; j < m; ++j) cfmac(acc, A22[(size_t)j * n + i], vbuf[j]);
+                wbuf[i] = cmul(tk, acc);
+            }
+            MSP_SYNC();
+            // alpha2 = -1/2 tau (p^H v)
+            double pre = 0.0, pim = 0.0;
+            MSP_PFOR(i, m)
+            {
+                cdbl t = cmulc(wbuf[i], vbuf[i]);
+                pre += t.re;
+                pim += t.im;
+            }
+            double sre = team_sum(pre, red);
+            double sim = team_sum(pim, red);
+            cdbl a2 = cmul(cmk(-0.5 * tk.re, -0.5 * tk.im), cmk(sre, sim));
+            MSP_PFOR(i, m) wbuf[i] = cadd(wbuf[i], cmul(a2, vbuf[i]));
+            MSP_SYNC();
+            // A22 -= v w^H + w v^H
+            MSP_PFOR(idx, m * m)
+            {
+                int i = idx / m, j = idx % m;
+                cdbl a = A22[(size_t)i * n + j];
+                cdbl vi = vbuf[i], wi = wbuf[i], vj = vbuf[j], wj = wbuf[j];
+                a = csub(a, cmul(vi, cconj(wj)));
+                a = csub(a, cmul(wi, cconj(vj)));
+                A22[(size_t)i * n + j] = a;
+            }
+            MSP_SYNC();
+        }
+#ifdef MSP_HOST_BUILD
+        d[k] = A[(size_t)k * n + k].re;
+        e[k] = beta;
+        tau[k] = tk;
+#else
+        if (MSP_TID == 0) {
+            d[k] = A[(size_t)k * n + k].re;
+            e[k] = beta;
+            tau[k] = tk;
+        }
+#endif
+        MSP_SYNC();
+    }
+#ifdef MSP_HOST_BUILD
+    d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+#else
+    if (MSP_TID == 0) d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+#endif
+    MSP_SYNC();
+}
+
+// ---------------------------------------------------------------------------
+// (2) Divide and conquer for the symmetric tridiagonal (d, e).
+//
+// The split tree halves [lo, hi) at mid = lo + (hi - lo)/2 down to single
+// entries, so every off-diagonal e_i is a split coupling: first
+// d_i -= |e_i| and d_{i+1} -= |e_i| for all i (Cuppen), then blocks are
+// merged bottom-up, all merges of one tree level together.  A merge of
+// T1 = Q1 D1 Q1^T and T2 = Q2 D2 Q2^T solves D + rho z z^T with
+// z = [last row of Q1, sign(e) first row of Q2]/sqrt(2), rho = 2|e|:
+//   - deflation of tiny rho|z_i| and of near-equal d pairs (Givens),
+//   - secular equation 1 + rho sum z_i^2/(d_i - lambda) = 0 per root,
+//     solved with the origin at the nearer pole by a two-pole rational
+//     model (Li's "middle way") safeguarded by bisection,
+//   - Loewner-recomputed z (Gu & Eisenstat) for orthogonal eigenvectors,
+//   - block multiply Q_new = Q_old S.
+// Q is n x n, row-major, column j = eigenvector j.
+
+struct DcWork {
+    double *Q0, *Q1, *S;       // n*n each
+    double *dv, *zv;           // current eigenvalues / merge z (by column)
+    double *ds, *zs;           // sorted d / z
+    int *perm;                 // sorted slot -> column
+    double *dl, *zl, *zh;      // kept (non-deflated) values, Loewner z
+    int *kcol;                 // kept slot -> column
+    int *korg;                 // origin pole per root
+    double *ktau;              // root offset from its origin
+    int *defl;                 // 1 = deflated (by sorted slot)
+    int *nkept;                // per block (indexed by lo)
+    int *rp, *rn;              // Givens pairs (columns), per block at lo..
+    double *rc, *rs;
+    int *nrot;
+    int *dfl;                  // deflated sorted slots, per block at lo..
+    double *item_v;            // merge items: kept roots then deflated slots
+    int *item_src;             // root index j >= 0, or ~slot for a deflated one
+    double *outv;              // merged eigenvalue per sorted output slot
+    int *outsrc;               // item source of each output column
+};
+
+MSP_HD int dc_depth(int n)
+{
+    int dep = 0, s = n;
+    while (s > 1) { s = s - s / 2; ++dep; }  // ceil halving
+    return dep;
+}
+
+// Block [lo, hi) containing position i at tree level L (root = level 0).
+MSP_HD void dc_block(int n, int L, int i, int &lo, int &hi)
+{
+    lo = 0;
+    hi = n;
+    for (int l = 0; l < L && hi - lo > 1; ++l) {
+        int mid = lo + (hi - lo) / 2;
+        if (i < mid) hi = mid; else lo = mid;
+    }
+}
+
+MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho, int j,
+                           int &org_out)
+{
+    // Root j of 1 + rho sum_i zl_i^2/(dl_i - x) = 0, dl ascending, rho > 0.
+    const double eps = 2.220446049250313e-16;
+    int org;
+    double lo, hi;
+    if (j < K - 1) {
+        double gap = dl[j + 1] - dl[j];
+        double mid = 0.5 * gap;
+        double f = 1.0;
+        for (int i = 0; i < K; ++i) f += rho * zl[i] * zl[i] / ((dl[i] - dl[j]) - mid);
+        if (f > 0.0) { org = j; lo = 0.0; hi = mid; }
+        else { org = j + 1; lo = -mid; hi = 0.0; }
+    } else {
+        double z2 = 0.0;
+        for (int i = 0; i < K; ++i) z2 += zl[i] * zl[i];
+        org = K - 1;
+        lo = 0.0;
+        hi = rho * z2;
+    }
+    org_out = org;
+    if (K == 1) return rho * zl[0] * zl[0];
+    const double dorg = dl[org];
+    // interior model uses the two poles j and j+1 around the root
+    const int pl = (j < K - 1) ? j : K - 1;
+    const int pr = (j < K - 1) ? j + 1 : -1;
+    double tau = 0.5 * (lo + hi);
+    for (int it = 0; it < 80; ++it) {
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+        for (int i = 0; i < K; ++i) {
+            double del = (dl[i] - dorg) - tau;
+            double t = rho * zl[i] / del;
+            if (i <= pl) { psi += t * zl[i]; dpsi += t * t / rho; }
+            else { phi += t * zl[i]; dphi += t * t / rho; }
+        }
+        double w = 1.0 + psi + phi;
+        if (w == 0.0) break;
+        if (w < 0.0) lo = tau; else hi = tau;
+        double eta;
+        const double Dl = (dl[pl] - dorg) - tau;
+        if (pr >= 0) {
+            const double Dr = (dl[pr] - dorg) - tau;
+            double c = w - Dl * dpsi - Dr * dphi;
+            double a = (Dl + Dr) * w - Dl * Dr * (dpsi + dphi);
+            double b = Dl * Dr * w;
+            double disc = sqrt(fabs(a * a - 4.0 * b * c));
+            if (c == 0.0) eta = b / a;
+            else if (a <= 0.0) eta = (a - disc) / (2.0 * c);
+            else eta = 2.0 * b / (a + disc);
+        } else {
+            // single pole on the left: c + s/(Dl - eta) = 0, s = Dl^2 psi'
+            double c = w - Dl * dpsi;
+            double s = Dl * Dl * dpsi;
+            eta = (c != 0.0) ? Dl + s / c : 0.5 * (lo + hi) - tau;
+        }
+        double tn = tau + eta;
+        bool inside = (pr >= 0) ? (tn > lo && tn < hi) : (tn > lo && tn <= hi);
+        if (!inside) tn = 0.5 * (lo + hi);
+        double step = fabs(tn - tau);
+        tau = tn;
+        double tol = 4.0 * eps * fmax(fabs(tau), 1e-300);
+        if (step <= tol || hi - lo <= tol) break;
+        double errb = 8.0 * eps * K * (1.0 + fabs(psi) + fabs(phi));
+        if (fabs(w) <= errb && step <= 64.0 * tol) break;
+    }
+    return tau;
+}
+
+// d (n): diagonal, overwritten by ascending eigenvalues; e (n-1): off-diagonal.
+// Q: on exit eigenvectors (points to W.Q0 or W.Q1, returned).
+MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *red)
+{
+    const double eps = 2.220446049250313e-16;
+    // scale to unit max norm
+    double part = 0.0;
+    MSP_PFOR(i, n) part = fmax(part, fabs(d[i]));
+    MSP_PFOR(i, n - 1) part = fmax(part, fabs(e[i]));
+    double scl = team_max(part, red);
+    if (scl == 0.0) scl = 1.0;
+    const double inv = 1.0 / scl;
+    MSP_PFOR(i, n)
+    {
+        double di = d[i] * inv;
+        if (i > 0) di -= fabs(e[i - 1] * inv);
+        if (i < n - 1) di -= fabs(e[i] * inv);
+        W.dv[i] = di;
+    }
+    double *Q = W.Q0, *Qn = W.Q1;
+    MSP_PFOR(idx, n * n) Q[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+    MSP_SYNC();
+
+    const int depth = dc_depth(n);
+    for (int L = depth - 1; L >= 0; --L) {
+        // --- z and sorted order per block (by position) ---
+        MSP_PFOR(i, n)
+        {
+            int lo, hi;
+            dc_block(n, L, i, lo, hi);
+            if (hi - lo >= 2) {
+                int mid = lo + (hi - lo) / 2;
+                double ec = e[mid - 1] * inv;
+                double zi;
+                if (i < mid) zi = Q[(size_t)(mid - 1) * n + i];
+                else zi = (ec < 0.0 ? -1.0 : 1.0) * Q[(size_t)mid * n + i];
+                W.zv[i] = zi * 0.7071067811865476;
+                // rank within the merged (two sorted halves) order
+                double di = W.dv[i];
+                int r;
+                if (i < mid) {
+                    int c = 0; // count in right half strictly less
+                    for (int k = mid; k < hi; ++k) c += (W.dv[k] < di);
+                    r = (i - lo) + c;
+                } else {
+                    int c = 0; // count in left half less or equal
+                    for (int k = lo; k < mid; ++k) c += (W.dv[k] <= di);
+                    r = (i - mid) + c;
+                }
+                W.perm[lo + r] = i;
+            }
+        }
+        MSP_SYNC();
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                W.ds[s] = W.dv[W.perm[s]];
+                W.zs[s] = W.zv[W.perm[s]];
+            }
+        }
+        MSP_SYNC();
+        // --- deflation scan, one thread per block ---
+        MSP_PFOR(lo0, n)
+        {
+            int lo, hi;
+            dc_block(n, L, lo0, lo, hi);
+            if (lo == lo0 && hi - lo >= 2) {
+                int mid = lo + (hi - lo) / 2;
+                double rho = 2.0 * fabs(e[mid - 1] * inv);
+                double dmax = 0.0, zmax = 0.0;
+                for (int s = lo; s < hi; ++s) {
+                    dmax = fmax(dmax, fabs(W.ds[s]));
+                    zmax = fmax(zmax, fabs(W.zs[s]));
+                }
+                double tol = 8.0 * eps * fmax(dmax, zmax);
+                int K = 0, nr = 0, nd = 0, pj = -1;
+                for (int s = lo; s < hi; ++s) {
+                    W.defl[s] = 0;
+                    if (rho * fabs(W.zs[s]) <= tol) {
+                        W.defl[s] = 1;
+                        W.dfl[lo + nd++] = s;
+                        continue;
+                    }
+                    if (pj < 0) { pj = s; continue; }
+                    double zs_ = W.zs[pj], zc = W.zs[s];
+                    double tu = sqrt(zs_ * zs_ + zc * zc);
+                    double c = zc / tu, sn = -zs_ / tu;
+                    double t = W.ds[s] - W.ds[pj];
+                    if (fabs(t * c * sn) <= tol) {
+                        // rotate columns perm[pj], perm[s]; pj deflates
+                        W.zs[s] = tu;
+                        W.zs[pj] = 0.0;
+                        double dp = W.ds[pj], dn = W.ds[s];
+                        W.ds[pj] = dp * c * c + dn * sn * sn;
+                        W.ds[s] = dp * sn * sn + dn * c * c;
+                        W.defl[pj] = 1;
+                        W.dfl[lo + nd++] = pj;
+                        W.rp[lo + nr] = W.perm[pj];
+                        W.rn[lo + nr] = W.perm[s];
+                        W.rc[lo + nr] = c;
+                        W.rs[lo + nr] = sn;
+                        ++nr;
+                        pj = s;
+                    } else {
+                        W.dl[lo + K] = W.ds[pj];
+                        W.zl[lo + K] = W.zs[pj];
+                        W.kcol[lo + K] = W.perm[pj];
+                        ++K;
+                        pj = s;
+                    }
+                }
+                if (pj >= 0) {
+                    W.dl[lo + K] = W.ds[pj];
+                    W.zl[lo + K] = W.zs[pj];
+                    W.kcol[lo + K] = W.perm[pj];
+                    ++K;
+                }
+                W.nkept[lo] = K;
+                W.nrot[lo] = nr;
+            }
+        }
+        MSP_SYNC();
+        // --- apply the deflation rotations to Q rows (in scan order) ---
+        MSP_PFOR(row, n)
+        {
+            int lo, hi;
+            dc_block(n, L, row, lo, hi);
+            if (hi - lo >= 2) {
+                int nr = W.nrot[lo];
+                for (int r = 0; r < nr; ++r) {
+                    int a = W.rp[lo + r], b = W.rn[lo + r];
+                    double c = W.rc[lo + r], s = W.rs[lo + r];
+                    double qa = Q[(size_t)row * n + a], qb = Q[(size_t)row * n + b];
+                    Q[(size_t)row * n + a] = c * qa + s * qb;
+                    Q[(size_t)row * n + b] = c * qb - s * qa;
+                }
+            }
+        }
+        // --- secular roots ---
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                int mid = lo + (hi - lo) / 2;
+                double rho = 2.0 * fabs(e[mid - 1] * inv);
+                int K = W.nkept[lo];
+                int j = s - lo;
+                if (j < K) {
+                    int org;
+                    double t = secular_root(K, W.dl + lo, W.zl + lo, rho, j, org);
+                    W.korg[s] = org;
+                    W.ktau[s] = t;
+                }
+            }
+        }
+        MSP_SYNC();
+        // --- Loewner z ---
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                int K = W.nkept[lo];
+                int i = s - lo;
+                if (i < K) {
+                    const double *dl = W.dl + lo;
+                    double di = dl[i];
+                    double wv = 1.0;
+                    for (int j = 0; j < K; ++j) {
+                        int org = W.korg[lo + j];
+                        double del = (di - dl[org]) - W.ktau[lo + j];  // d_i - lambda_j
+                        if (j == i) wv *= del;
+                        else wv *= del / (di - dl[j]);
+                    }
+                    double zz = sqrt(fabs(wv));
+                    W.zh[s] = (W.zl[s] < 0.0) ? -zz : zz;
+                }
+            }
+        }
+        MSP_SYNC();
+        // --- eigenvectors of the rank-one problem: S[i][j] (block-local square) ---
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                int K = W.nkept[lo];
+                int j = s - lo;
+                if (j < K) {
+                    const double *dl = W.dl + lo;
+                    int org = W.korg[s];
+                    double tj = W.ktau[s];
+                    double nrm = 0.0;
+                    for (int i = 0; i < K; ++i) {
+                        double del = (dl[i] - dl[org]) - tj;
+                        double u = W.zh[lo + i] / del;
+                        W.S[(size_t)(lo + i) * n + (lo + j)] = u;
+                        nrm += u * u;
+                    }
+                    nrm = 1.0 / sqrt(nrm);
+                    for (int i = 0; i < K; ++i) W.S[(size_t)(lo + i) * n + (lo + j)] *= nrm;
+                }
+            }
+        }
+        MSP_SYNC();
+        // --- merge items (kept roots, then deflated slots) ---
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                int K = W.nkept[lo];
+                int t = s - lo;
+                if (t < K) {
+                    W.item_v[s] = W.dl[lo + W.korg[s]] + W.ktau[s];
+                    W.item_src[s] = t;
+                } else {
+                    int sl = W.dfl[lo + (t - K)];
+                    W.item_v[s] = W.ds[sl];
+                    W.item_src[s] = ~sl;
+                }
+            }
+        }
+        MSP_SYNC();
+        // --- ascending order of the merged eigenvalues, ties by item index ---
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) {
+                double v = W.item_v[s];
+                int r = 0;
+                for (int t = lo; t < hi; ++t) {
+                    double vt = W.item_v[t];
+                    r += (vt < v || (vt == v && t < s));
+                }
+                W.outv[lo + r] = v;
+                W.outsrc[lo + r] = W.item_src[s];
+            }
+        }
+        MSP_SYNC();
+        // --- new Q columns ---
+        MSP_PFOR(idx, n * n)
+        {
+            int row = idx / n, col = idx % n;
+            int lo, hi;
+            dc_block(n, L, col, lo, hi);
+            if (hi - lo >= 2) {
+                int src = W.outsrc[col];
+                double val = 0.0;
+                if (row >= lo && row < hi) {
+                    if (src >= 0) {
+                        int K = W.nkept[lo];
+                        for (int i = 0; i < K; ++i)
+                            val += Q[(size_t)row * n + W.kcol[lo + i]] * W.S[(size_t)(lo + i) * n + (lo + src)];
+                    } else {
+                        val = Q[(size_t)row * n + W.perm[~src]];
+                    }
+                }
+                Qn[(size_t)row * n + col] = val;
+            } else {
+                Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
+            }
+        }
+        MSP_SYNC();
+        MSP_PFOR(s, n)
+        {
+            int lo, hi;
+            dc_block(n, L, s, lo, hi);
+            if (hi - lo >= 2) W.dv[s] = W.outv[s];
+        }
+        MSP_SYNC();
+        double *t = Q;
+        Q = Qn;
+        Qn = t;
+    }
+    MSP_PFOR(i, n) d[i] = W.dv[i] * scl;
+    MSP_SYNC();
+    return Q;
+}
+
+// ---------------------------------------------------------------------------
+// (3) U = Q Z, Q = H(0) H(1) ... H(n-2), one column chunk [c0, c0+cw) held
+// in LDS (Uc: n x cw complex, row stride cw).  Z is real row-major; U complex
+// row-major (row stride n).  A holds the reflector tails (tridiag_team).
+// Chunks are independent, so the device runs one workgroup per chunk.
+MSP_HD void backtransform_chunk_team(int n, const cdbl *A, const cdbl *tau, const double *Z,
+                                     cdbl *U, cdbl *Uc, int cw, int c0, cdbl *wsum)
+{
+    const int cwk = (n - c0 < cw) ? n - c0 : cw;
+    MSP_PFOR(idx, n * cwk)
+    {
+        int i = idx / cwk, c = idx % cwk;
+        Uc[i * cw + c] = cmk(Z[(size_t)i * n + c0 + c], 0.0);
+    }
+    MSP_SYNC();
+    for (int k = n - 2; k >= 0; --k) {
+        const cdbl tk = tau[k];
+        if (tk.re == 0.0 && tk.im == 0.0) continue;
+        const int m = n - k - 1;
+        const cdbl *col = A + (size_t)(k + 1) * n + k;
+        // w_c = tau sum_i conj(v_i) U[k+1+i][c]
+        MSP_PFOR(c, cwk)
+        {
+            cdbl acc = Uc[(k + 1) * cw + c];
+            for (int i = 1; i < m; ++i) cfmac(acc, col[(size_t)i * n], Uc[(k + 1 + i) * cw + c]);
+            wsum[c] = cmul(tk, acc);
+        }
+        MSP_SYNC();
+        MSP_PFOR(idx, m * cwk)
+        {
+            int i = idx / cwk, c = idx % cwk;
+            cdbl vi = (i == 0) ? cmk(1.0, 0.0) : col[(size_t)i * n];
+            cdbl u = Uc[(k + 1 + i) * cw + c];
+            Uc[(k + 1 + i) * cw + c] = csub(u, cmul(vi, wsum[c]));
+        }
+        MSP_SYNC();
+    }
+    MSP_PFOR(idx, n * cwk)
+    {
+        int i = idx / cwk, c = idx % cwk;
+        U[(size_t)i * n + c0 + c] = Uc[i * cw + c];
+    }
+    MSP_SYNC();
+}
+
+MSP_HD void backtransform_team(int n, const cdbl *A, const cdbl *tau, const double *Z,
+                               cdbl *U, cdbl *Uc, int cw, cdbl *wsum)
+{
+    for (int c0 = 0; c0 < n; c0 += cw) backtransform_chunk_team(n, A, tau, Z, U, Uc, cw, c0, wsum);
+}
+
+// Doubles of per-matrix global workspace used by the divide-and-conquer path:
+// e (n), tau (2n), DcWork doubles (3n^2 + 12n) and ints (11n, padded).
+MSP_HD size_t dc_work_doubles(int n)
+{
+    return (size_t)3 * n * n + 15 * (size_t)n + 6 * (size_t)n + 8;
+}
+
+// Carve a DcWork (and e, tau) out of one contiguous per-matrix block.
+MSP_HD DcWork dc_carve(int n, double *base, double **e, cdbl **tau)
+{
+    DcWork W;
+    double *p = base;
+    *e = p; p += n;
+    *tau = (cdbl *)p; p += 2 * n;
+    W.Q0 = p; p += (size_t)n * n;
+    W.Q1 = p; p += (size_t)n * n;
+    W.S = p; p += (size_t)n * n;
+    W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
+    W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
+    W.rc = p; p += n; W.rs = p; p += n; W.item_v = p; p += n; W.outv = p; p += n;
+    int *q = (int *)p;
+    W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
+    W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
+    W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n;
+    return W;
+}
+
+// The eigenvector matrix stedc_team leaves after dc_depth(n) buffer swaps.
+MSP_HD double *dc_result(int n, const DcWork &W)
+{
+    return (dc_depth(n) & 1) ? W.Q1 : W.Q0;
+}

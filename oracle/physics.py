@@ -18,10 +18,21 @@ kind "scalar" (tensor: float), "linear" (tensor: 3-vector, on spin i) or
 S_i^a S_i^b, as for quadrupolar terms).
 """
 
+import functools
+
 import numpy as np
 import scipy.constants as cnst
 
 from . import spin
+
+
+@functools.lru_cache(maxsize=64)
+def _spin_ops(Is):
+    """Full-space S_i^a for every spin, built once per spin list like the
+    reference's cached single-spin operators (experiment.py:105-111)."""
+    ops = np.array([[spin.spin_op(list(Is), i, a) for a in _XYZ] for i in range(len(Is))])
+    ops.setflags(write=False)
+    return ops
 
 ELEC_GAMMA = -28024.9514242                # constants.py:12
 MU_GAMMA = -(ELEC_GAMMA / 206.7669883)     # constants.py:13
@@ -85,11 +96,12 @@ def hamiltonian(Is, terms):
 def zeeman(Is, gammas, B):
     """Sum_i gamma_i B.S_i (ExperimentRunner.Hz, experiment.py:238-254)."""
     B = np.asarray(B, dtype=float)
-    n = int(np.prod([int(round(2 * I + 1)) for I in Is]))
+    ops = _spin_ops(tuple(Is))
+    n = ops.shape[-1]
     H = np.zeros((n, n), dtype=complex)
     for i, g in enumerate(gammas):
         for a in range(3):
-            H += spin.spin_op(Is, i, _XYZ[a]) * (B[a] * g)
+            H += ops[i, a] * (B[a] * g)
     return H
 
 
@@ -130,7 +142,8 @@ def rho0_factors(Is, gammas, muon_index, B, p, T):
 
 def muon_operator(Is, muon_index, v):
     """v.S_mu in the full space (spinsys.py:707-732)."""
-    return sum(v[a] * spin.spin_op(Is, muon_index, _XYZ[a]) for a in range(3))
+    ops = _spin_ops(tuple(Is))
+    return sum(v[a] * ops[muon_index, a] for a in range(3))
 
 
 def sigma_mu(v):

@@ -31,11 +31,15 @@ def load_config(q, B, Bint, p, T, muon_index):
     return Br, pr, fast
 
 
-def run_single(system, y, times, Br, pr, T, w, fast):
+def run_single(system, y, times, Br, pr, T, w, fast, Hsys=None):
     """One instance: Htot -> fast_evolve / evolve / Lindblad / integral,
-    Re(data) * w (experiment.py:434-498)."""
+    Re(data) * w (experiment.py:434-498).  Hsys, when given, is the
+    field-independent part, built once per system as ExperimentRunner does
+    (experiment.py:119)."""
     Is = system.Is
-    H = physics.hamiltonian(Is, system.terms) + physics.zeeman(Is, system.gammas, Br)
+    if Hsys is None:
+        Hsys = physics.hamiltonian(Is, system.terms)
+    H = Hsys + physics.zeeman(Is, system.gammas, Br)
     dissip = getattr(system, "dissip", {}) or {}
     pop = physics.muon_operator(Is, system.muon_index, pr)
     if dissip:
@@ -66,11 +70,15 @@ def instance_rows(systems, batch, indices=None):
     if indices is None:
         indices = range(len(batch.w))
     rows = []
+    hsys = {}
     for k in indices:
-        s = systems[int(batch.sys_id[k])]
+        sid = int(batch.sys_id[k])
+        s = systems[sid]
+        if sid not in hsys:
+            hsys[sid] = physics.hamiltonian(s.Is, s.terms)
         Br, pr, fast = load_config(batch.q[k], batch.B[k], batch.Bint[k], batch.p[k],
                                    batch.T[k], s.muon_index)
-        r = run_single(s, batch.y, batch.times, Br, pr, batch.T[k], batch.w[k], fast)
+        r = run_single(s, batch.y, batch.times, Br, pr, batch.T[k], batch.w[k], fast, hsys[sid])
         rows.append(np.atleast_1d(r))
     return np.array(rows)
 

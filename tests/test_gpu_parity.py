@@ -1,0 +1,204 @@
+"""Parity of the HIP path with the FP64 oracle (runs on an MI355X).
+
+Tolerances: the plain FP64 path must reproduce the oracle's P(t) / integral
+to max |dP| <= 1e-10 (absolute, P in [-0.5, 0.5]); stage-level checks use
+the tolerances written next to each assertion.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import dynamics, physics
+from oracle import pipeline as opipe
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-10
+
+
+def _herm(rng, n, scale=1.0):
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    return scale * (X + X.conj().T) / 2
+
+
+@pytest.fixture(scope="module")
+def eng(gpu_lib):
+    from muspinsim_amd.engine import SpinEngine
+    return SpinEngine("cuda:0")
+
+
+def _dev(a, dt=torch.complex128):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dt).cuda()
+
+
+@pytest.mark.parametrize("n,method", [(1, 1), (2, 1), (3, 1), (8, 1), (16, 1), (33, 1), (64, 1),
+                                      (2, 2), (5, 2), (16, 2), (65, 2), (128, 2), (192, 2)])
+def test_eigh_random(eng, n, method):
+    rng = np.random.default_rng(n)
+    Hs = np.array([_herm(rng, n) for _ in range(3)])
+    E, U, info = eng.eigh(_dev(Hs), method=method)
+    torch.cuda.synchronize()
+    E, U = E.cpu().numpy(), U.cpu().numpy()
+    assert np.all(info.cpu().numpy() == 0)
+    for k in range(3):
+        Er = np.linalg.eigvalsh(Hs[k])
+        nrm = max(1.0, np.abs(Er).max())
+        # backward-stable eigensolver: errors of order n eps ||H||
+        assert np.abs(E[k] - Er).max() <= 1e-13 * n * nrm
+        assert np.abs(Hs[k] @ U[k] - U[k] * E[k]).max() <= 1e-13 * n * nrm
+        assert np.abs(U[k].conj().T @ U[k] - np.eye(n)).max() <= 1e-13 * n
+        assert np.all(np.diff(E[k]) >= 0)
+
+
+def test_eigh_degenerate_and_zero(eng):
+    """Exactly degenerate spectra (zero field, isotropic hyperfine) and the
+    zero matrix."""
+    Is = [0.5, 0.5, 0.5, 0.5]
+    H = physics.hamiltonian(Is, [("bilinear", 0, 1, np.eye(3) * 100)])
+    for method in (1, 2):
+        E, U, info = eng.eigh(_dev(np.array([H, np.zeros_like(H)])), method=method)
+        E, U = E.cpu().numpy(), U.cpu().numpy()
+        assert np.abs(E[0] - np.linalg.eigvalsh(H)).max() < 1e-12
+        assert np.abs(H @ U[0] - U[0] * E[0]).max() < 1e-12
+        assert np.abs(U[1].conj().T @ U[1] - np.eye(16)).max() < 1e-14
+        assert np.all(E[1] == 0)
+
+
+def _build(eng, systems, batch, idx):
+    from muspinsim_amd import _native as N
+    from muspinsim_amd.engine import pack_systems
+    L = N.lib()
+    sysarr, termarr = pack_systems(systems)
+    dsys, dterms = eng._bytes_view(sysarr), eng._bytes_view(termarr)
+    n = systems[0].n
+    nb = len(idx)
+    Br = torch.empty((nb, 3), dtype=torch.float64, device="cuda")
+    pr = torch.empty_like(Br)
+    q, B, Bi, p = (_dev(getattr(batch, a)[idx], torch.float64) for a in ("q", "B", "Bint", "p"))
+    sid = _dev(batch.sys_id[idx].astype(np.int32), torch.int32)
+    N.check(L.msp_rotate_fields(nb, q.data_ptr(), B.data_ptr(), Bi.data_ptr(), p.data_ptr(),
+                                Br.data_ptr(), pr.data_ptr(), eng.stream()))
+    H = torch.empty((nb, n, n), dtype=torch.complex128, device="cuda")
+    N.check(L.msp_build_h(n, nb, dsys.data_ptr(), dterms.data_ptr(), sid.data_ptr(), Br.data_ptr(),
+                          None, 1, H.data_ptr(), eng.stream()))
+    torch.cuda.synchronize()
+    return H.cpu().numpy(), Br.cpu().numpy(), pr.cpu().numpy()
+
+
+def test_build_h_matches_oracle(eng):
+    from muspinsim_amd import workloads as W
+    for systems, batch in (W.c1_zero_field_powder(nori=6), W.c2_alc_sweep(nfields=3, nori=2)):
+        idx = np.arange(len(batch))
+        H, Br, pr = _build(eng, systems, batch, idx)
+        s = systems[0]
+        for k in idx:
+            Brh, prh, _ = opipe.load_config(batch.q[k], batch.B[k], batch.Bint[k], batch.p[k],
+                                            batch.T[k], 0)
+            assert np.abs(Br[k] - Brh).max() < 1e-14 * max(1, np.abs(Brh).max())
+            assert np.abs(pr[k] - prh).max() < 1e-15
+            Href = physics.hamiltonian(s.Is, s.terms) + physics.zeeman(s.Is, s.gammas, Brh)
+            assert np.abs(H[k] - Href).max() <= 1e-12 * np.abs(Href).max()
+
+
+def test_gemm_cn(eng):
+    rng = np.random.default_rng(7)
+    for n in (1, 5, 32, 47, 192):
+        A = rng.normal(size=(2, n, n)) + 1j * rng.normal(size=(2, n, n))
+        B = rng.normal(size=(2, n, n)) + 1j * rng.normal(size=(2, n, n))
+        C = eng.gemm_cn(_dev(A), _dev(B)).cpu().numpy()
+        ref = np.conj(np.swapaxes(A, 1, 2)) @ B
+        assert np.abs(C - ref).max() <= 1e-13 * n
+
+
+def test_timesum_and_integral(eng):
+    rng = np.random.default_rng(3)
+    for n, T in ((4, 7), (16, 130), (70, 65)):
+        E = np.sort(rng.normal(size=(2, n)) * 50)
+        Cc = rng.normal(size=(2, n, n)) + 1j * rng.normal(size=(2, n, n))
+        t = np.linspace(0, 10, T)
+        w = np.array([0.7, 1.3])
+        out = eng.timesum(_dev(E, torch.float64), _dev(Cc), _dev(t, torch.float64), False,
+                          _dev(w, torch.float64)).cpu().numpy()
+        outc = eng.timesum(_dev(E, torch.float64), _dev(Cc), _dev(t, torch.float64), False,
+                           None, True).cpu().numpy()
+        for b in range(2):
+            ll = -2j * np.pi * (E[b][:, None] - E[b][None, :])
+            ref = np.array([np.sum(np.exp(ll * tt) * Cc[b]) for tt in t])
+            assert np.abs(outc[b] - ref).max() <= 1e-12 * np.abs(Cc[b]).sum()
+            assert np.abs(out[b] - w[b] * ref.real).max() <= 1e-12 * np.abs(Cc[b]).sum()
+        integ = eng.integral(_dev(E, torch.float64), _dev(Cc), False, 2.0, 0.5, None,
+                             True).cpu().numpy()
+        for b in range(2):
+            ref = np.sum(Cc[b] / (0.5 + 2j * np.pi * (E[b][:, None] - E[b][None, :]))) * 0.5
+            assert abs(integ[b] - ref) <= 1e-13 * np.abs(Cc[b]).sum()
+
+
+def _end_to_end(systems, batch, eng, tol=PARITY):
+    res = eng.run(systems, batch)
+    ref = opipe.run(systems, batch)
+    err = np.abs(res - ref).max()
+    assert err <= tol, f"max |dP| = {err:.3e}"
+    return err
+
+
+def test_c1_full_parity(eng):
+    """Zero-field powder, n=16, 100 orientations x 1000 times (fast path)."""
+    from muspinsim_amd import workloads as W
+    systems, batch = W.c1_zero_field_powder()
+    _end_to_end(systems, batch, eng)
+
+
+def test_c2_subset_parity(eng):
+    """ALC integral sweep, n=192, a 12 field x 5 orientation subset."""
+    from muspinsim_amd import workloads as W
+    systems, batch = W.c2_alc_sweep(nfields=12, nori=5)
+    _end_to_end(systems, batch, eng)
+
+
+def test_c3_reduced_parity(eng):
+    """n=512 transverse field, 3 orientations x 300 times in 0-20 us."""
+    from muspinsim_amd import workloads as W
+    systems, batch = W.c3_tf_stress(nori=3, ntimes=300)
+    _end_to_end(systems, batch, eng)
+
+
+def test_c5_reduced_parity(eng):
+    """n=64 (Jacobi bucket), 16 orientations x 500 times."""
+    from muspinsim_amd import workloads as W
+    systems, batch = W.c5_fit(nori=16, ntimes=500)
+    _end_to_end(systems, batch, eng)
+
+
+def test_c4_small_buckets_parity(eng):
+    """Uneven batch: the first 60 instances (n = 4 .. 256) in one call."""
+    from muspinsim_amd import workloads as W
+    systems, batch = W.c4_uneven_batch(ninst=60)
+    keep = [k for k in range(60) if systems[k].n <= 256]
+    from muspinsim_amd.engine import InstanceBatch
+    sub = InstanceBatch(q=batch.q[keep], B=batch.B[keep], Bint=batch.Bint[keep], p=batch.p[keep],
+                        T=batch.T[keep], w=batch.w[keep], slot=batch.slot[keep],
+                        sys_id=batch.sys_id[keep], times=batch.times, y=batch.y,
+                        n_slots=1, avg_N=len(keep))
+    _end_to_end(systems, sub, eng)
+
+
+def test_general_route_finite_temperature(eng):
+    """Finite T in a field: rho0 = kron of thermal states, second basis change
+    (experiment.py:169-236, hamiltonian.py:40-107); muon not first."""
+    from muspinsim_amd.engine import InstanceBatch, SystemDesc
+    ge, gmu = physics.ELEC_GAMMA, physics.MU_GAMMA
+    s = SystemDesc([0.5, 0.5, 1.0], [ge, gmu, 3.0777],
+                   [("bilinear", 1, 0, np.eye(3) * 50 + np.diag([1, 2, -3.0])),
+                    ("bilinear", 2, 0, np.eye(3) * 5),
+                    ("bilinear", 2, 2, np.diag([0.3, 0.2, -0.5]))], muon_index=1)
+    n = 4
+    q = np.array([[1.0, 0, 0, 0], [0.9, 0.1, 0.3, 0.2], [0.5, 0.5, 0.5, 0.5], [0.2, 0.9, 0.1, 0.3]])
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    for y, times in (("asymmetry", np.linspace(0, 5, 77)), ("integral", np.zeros(1))):
+        batch = InstanceBatch(q=q, B=np.tile([0.01, 0.0, 0.05], (n, 1)), Bint=np.zeros((n, 3)),
+                              p=np.tile([1.0, 0, 0], (n, 1)), T=np.array([0.5, 1.0, 2.0, 5.0]),
+                              w=np.ones(n), slot=np.array([0, 1, 0, 1]),
+                              sys_id=np.zeros(n, dtype=np.int64), times=times, y=y,
+                              n_slots=2, avg_N=2)
+        _end_to_end([s], batch, eng)
