@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4000,
                     help="instances of the sweep timed on the host (0 = skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -114,7 +115,7 @@ def main():
     from muspinsim_amd import _native as N
     from muspinsim_amd.engine import SpinEngine
 
-    eng = SpinEngine("cuda:0")
+    eng = SpinEngine("cuda:0", max_chunk=args.chunk) if args.chunk else SpinEngine("cuda:0")
     plan = eng.prepare(systems, batch)
     for _ in range(args.warmup):
         eng.execute(plan)

@@ -43,19 +43,20 @@ struct DcAlloc {
     std::vector<double> dbl;
     std::vector<int> in;
     DcWork W;
-    DcAlloc(int n) : dbl((size_t)3 * n * n + 12 * (size_t)n), in(16 * (size_t)n)
+    DcAlloc(int n) : dbl((size_t)4 * n * n + 12 * (size_t)n), in(16 * (size_t)n)
     {
         double *p = dbl.data();
         W.Q0 = p; p += (size_t)n * n;
         W.Q1 = p; p += (size_t)n * n;
         W.S = p; p += (size_t)n * n;
+        W.QK = p; p += (size_t)n * n;
         W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
         W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
         W.rc = p; p += n; W.rs = p; p += n; W.item_v = p; p += n; W.outv = p; p += n;
         int *q = in.data();
         W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
         W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
-        W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n;
+        W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
     }
 };
 
@@ -69,7 +70,39 @@ int hc_stedc(int n, double *d, const double *e, double *Z)
     return 0;
 }
 
+int hc_tridiag_blocked(int n, double *A, double *d, double *e, double *tau, int nb)
+{
+    std::vector<cdbl> vb(n), wb(n), Vp((size_t)n * nb), Wp((size_t)n * nb), y(2 * nb);
+    double red[1];
+    tridiag_blocked_team(n, (cdbl *)A, d, e, (cdbl *)tau, nb, Vp.data(), Wp.data(), vb.data(),
+                         wb.data(), y.data(), red);
+    return 0;
+}
+
 // Full path: tridiagonalise, divide and conquer, back-transform.
+int hc_eigh_dc_blocked(int n, const double *H, double *E, double *U, int cw, int nb)
+{
+    std::vector<cdbl> A((size_t)n * n);
+    memcpy(A.data(), H, sizeof(cdbl) * n * n);
+    std::vector<double> e(n);
+    std::vector<cdbl> tau(n), vb(n), wb(n);
+    double red[1];
+    tridiag_team(n, A.data(), E, e.data(), tau.data(), vb.data(), wb.data(), red);
+    DcAlloc al(n);
+    double *Q = stedc_team(n, E, e.data(), al.W, red);
+    int nblk = (n - 1 + nb - 1) / nb;
+    std::vector<cdbl> T((size_t)(nblk > 0 ? nblk : 1) * nb * nb), G((size_t)nb * nb);
+    for (int b = 0; b < nblk; ++b) {
+        int k0 = b * nb, nbk = (n - 1 - k0 < nb) ? n - 1 - k0 : nb;
+        larft_block_team(n, A.data(), tau.data(), k0, nbk, nb, T.data() + (size_t)b * nb * nb, G.data());
+    }
+    std::vector<cdbl> Uc((size_t)n * (cw + 1)), W((size_t)nb * cw), W2((size_t)nb * cw);
+    for (int c0 = 0; c0 < n; c0 += cw)
+        backtransform_blocked_team(n, A.data(), T.data(), Q, (cdbl *)U, Uc.data(), cw + 1, cw, c0, nb,
+                                   W.data(), W2.data());
+    return 0;
+}
+
 int hc_eigh_dc(int n, const double *H, double *E, double *U, int cw)
 {
     std::vector<cdbl> A((size_t)n * n);

@@ -191,6 +191,24 @@ size_t jacobi_lds_bytes(int n)
 
 // ---------------------------------------------------------------------------
 // Divide-and-conquer path, three kernels per batch.
+__global__ void __launch_bounds__(256) k_tridiag2(int n, double *H, double *E, double *work)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.x;
+    cdbl *vb = (cdbl *)smem;
+    cdbl *wb = vb + n;
+    cdbl *y = wb + n;
+    double *red = (double *)(y + 2 * MSP_TD_NB);
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    double *e;
+    cdbl *tau;
+    dc_carve(n, base, &e, &tau);
+    cdbl *Vp = dc_panels(n, base);
+    cdbl *Wp = Vp + (size_t)n * MSP_TD_NB;
+    tridiag_blocked_team(n, (cdbl *)H + (size_t)b * n * n, E + (size_t)b * n, e, tau, MSP_TD_NB,
+                         Vp, Wp, vb, wb, y, red);
+}
+
 __global__ void k_tridiag(int n, double *H, double *E, double *work)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -204,15 +222,52 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
     tridiag_team(n, (cdbl *)H + (size_t)b * n * n, E + (size_t)b * n, e, tau, vb, wb, red);
 }
 
-__global__ void k_stedc(int n, double *E, double *work, int32_t *info)
+__global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info)
 {
-    __shared__ double red[16];
+    __shared__ double red[2 * 32 * 33];
     const int b = blockIdx.x;
     double *e;
     cdbl *tau;
     DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
     stedc_team(n, E + (size_t)b * n, e, W, red);
     if (threadIdx.x == 0) info[b] = 0;
+}
+
+// T factors of every reflector block: grid (blocks, instances).
+__global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *work)
+{
+    __shared__ cdbl G[MSP_BT_NB * MSP_BT_NB];
+    __shared__ cdbl T[MSP_BT_NB * MSP_BT_NB];
+    const int b = blockIdx.y, blk = blockIdx.x;
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    double *e;
+    cdbl *tau;
+    dc_carve(n, base, &e, &tau);
+    const int k0 = blk * MSP_BT_NB;
+    const int nbk = (n - 1 - k0 < MSP_BT_NB) ? n - 1 - k0 : MSP_BT_NB;
+    larft_block_team(n, (const cdbl *)H + (size_t)b * n * n, tau, k0, nbk, MSP_BT_NB, T, G);
+    cdbl *Tout = dc_tblocks(n, base) + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
+    MSP_PFOR(idx, MSP_BT_NB * MSP_BT_NB) Tout[idx] = T[idx];
+}
+
+// U = Q Z on column chunks of width cw: grid (chunks, instances).
+__global__ void __launch_bounds__(256) k_backtransform2(int n, const double *H, double *work,
+                                                        double *U, int cw)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.y;
+    const int c0 = blockIdx.x * cw;
+    const int ldu = cw + 1;
+    cdbl *Uc = (cdbl *)smem;
+    cdbl *W = Uc + (size_t)n * ldu;
+    cdbl *W2 = W + MSP_BT_NB * cw;
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    double *e;
+    cdbl *tau;
+    DcWork Wk = dc_carve(n, base, &e, &tau);
+    backtransform_blocked_team(n, (const cdbl *)H + (size_t)b * n * n, dc_tblocks(n, base),
+                               dc_result(n, Wk), (cdbl *)U + (size_t)b * n * n, Uc, ldu, cw, c0,
+                               MSP_BT_NB, W, W2);
 }
 
 __global__ void k_backtransform(int n, const double *H, double *work, double *U, int cw)
@@ -592,6 +647,15 @@ static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) 
 
 size_t dc_workspace_doubles(int n) { return dc_work_doubles(n); }
 
+// Blocked back-transformation chunk width: U chunk (n x (cw+1)) plus the two
+// nb x cw block products within ~76 KB of LDS (two workgroups per CU).
+int bt2_chunk_width(int n)
+{
+    int cw = 64;
+    while (cw > 2 && ((size_t)n * (cw + 1) + 2 * MSP_BT_NB * cw) * sizeof(cdbl) > 76 * 1024) cw /= 2;
+    return cw;
+}
+
 // Back-transformation chunk width: as many columns as fit ~128 KB of LDS.
 int dc_chunk_width(int n)
 {
@@ -641,25 +705,38 @@ hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, doub
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
                           double *work, hipStream_t s)
 {
-    size_t lds1 = 2 * sizeof(cdbl) * (size_t)n + 16 * sizeof(double);
-    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-    if (err != hipSuccess) return err;
-    { ProfScope ps_("k_tridiag", s);
-    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(256), lds1, s, n, H, E, work);
+    hipError_t err = hipSuccess;
+    if (n <= 512) {
+        ProfScope ps_("k_tridiag", s);
+        err = launch_tridiag_lp(n, batch, H, E, work, s);
+    } else {
+        size_t lds1 = sizeof(cdbl) * 2 * (size_t)n + 16 * sizeof(double);
+        err = hipFuncSetAttribute((const void *)k_tridiag,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+        if (err != hipSuccess) return err;
+        ProfScope ps_("k_tridiag", s);
+        hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(256), lds1, s, n, H, E, work);
+        err = hipGetLastError();
     }
+    if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
     { ProfScope ps_("k_stedc", s);
     hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), 0, s, n, E, work, info);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    const int cw = dc_chunk_width(n);
-    size_t lds3 = sizeof(cdbl) * ((size_t)n * cw + cw);
-    err = hipFuncSetAttribute((const void *)k_backtransform,
+    if (bt_nblocks(n) > 0) {
+        ProfScope ps_("k_larft", s);
+        hipLaunchKernelGGL(k_larft, dim3(bt_nblocks(n), batch), dim3(256), 0, s, n,
+                           (const double *)H, work);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int cw = bt2_chunk_width(n);
+    size_t lds3 = sizeof(cdbl) * ((size_t)n * (cw + 1) + 2 * MSP_BT_NB * cw);
+    err = hipFuncSetAttribute((const void *)k_backtransform2,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);
     if (err != hipSuccess) return err;
     { ProfScope ps_("k_backtransform", s);
-    hipLaunchKernelGGL(k_backtransform, dim3(cdiv(n, cw), batch), dim3(256), lds3, s, n,
+    hipLaunchKernelGGL(k_backtransform2, dim3(cdiv(n, cw), batch), dim3(256), lds3, s, n,
                        (const double *)H, work, U, cw);
     }
     return hipGetLastError();

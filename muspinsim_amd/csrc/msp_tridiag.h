@@ -133,7 +133,7 @@ MSP_HD void tridiag_team(int n, cdbl *A, double *d, double *e, cdbl *tau,
 // Q is n x n, row-major, column j = eigenvector j.
 
 struct DcWork {
-    double *Q0, *Q1, *S;       // n*n each
+    double *Q0, *Q1, *S, *QK;  // n*n each (QK: gathered kept columns)
     double *dv, *zv;           // current eigenvalues / merge z (by column)
     double *ds, *zs;           // sorted d / z
     int *perm;                 // sorted slot -> column
@@ -151,6 +151,7 @@ struct DcWork {
     int *item_src;             // root index j >= 0, or ~slot for a deflated one
     double *outv;              // merged eigenvalue per sorted output slot
     int *outsrc;               // item source of each output column
+    int *rootpos;              // kept root j -> output column
 };
 
 MSP_HD int dc_depth(int n)
@@ -238,6 +239,91 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
         if (fabs(w) <= errb && step <= 64.0 * tol) break;
     }
     return tau;
+}
+
+// Qn[lo+r][rootpos[lo+j]] = sum_i QK[lo+r][lo+i] S[lo+i][lo+j] for every merge
+// block of level L.  Large blocks go through 32 x 32 LDS tiles (the team's
+// `red` scratch must hold 2 * 32 * 33 doubles); small ones are direct dots.
+MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red)
+{
+    // small blocks: one output per iteration
+    MSP_PFOR(idx, n * n)
+    {
+        int row = idx / n, col = idx % n;
+        int lo, hi;
+        dc_block(n, L, col, lo, hi);
+        if (hi - lo >= 2 && hi - lo < 48 && row >= lo && row < hi) {
+            int K = W.nkept[lo];
+            int j = col - lo;
+            if (j < K) {
+                double acc = 0.0;
+                for (int i = 0; i < K; ++i)
+                    acc += W.QK[(size_t)row * n + lo + i] * W.S[(size_t)(lo + i) * n + lo + j];
+                Qn[(size_t)row * n + W.rootpos[lo + j]] = acc;
+            }
+        }
+    }
+    // large blocks: tiled, one block after another
+    for (int lo = 0; lo < n;) {
+        int blo, bhi;
+        dc_block(n, L, lo, blo, bhi);
+        const int sz = bhi - blo;
+        if (sz >= 48) {
+            const int K = W.nkept[blo];
+            double *As = red;            // 32 x 33 (rows x k)
+            double *Bs = red + 32 * 33;  // 32 x 33 (k x cols)
+            for (int r0 = 0; r0 < sz; r0 += 32)
+                for (int j0 = 0; j0 < K; j0 += 32) {
+#ifdef MSP_HOST_BUILD
+                    for (int rr = 0; rr < 32; ++rr)
+                        for (int jj = 0; jj < 32; ++jj) {
+                            int r = r0 + rr, j = j0 + jj;
+                            if (r >= sz || j >= K) continue;
+                            double acc = 0.0;
+                            for (int i = 0; i < K; ++i)
+                                acc += W.QK[(size_t)(blo + r) * n + blo + i] * W.S[(size_t)(blo + i) * n + blo + j];
+                            Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
+                        }
+                    (void)As; (void)Bs;
+#else
+                    const int tx = MSP_TID % 16, ty = MSP_TID / 16;   // 256 threads
+                    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                    for (int i0 = 0; i0 < K; i0 += 32) {
+                        for (int l = MSP_TID; l < 32 * 32; l += MSP_NT) {
+                            int a = l / 32, b = l % 32;
+                            int r = r0 + a, i = i0 + b;
+                            As[a * 33 + b] = (r < sz && i < K) ? W.QK[(size_t)(blo + r) * n + blo + i] : 0.0;
+                            int ii = i0 + a, j = j0 + b;
+                            Bs[a * 33 + b] = (ii < K && j < K) ? W.S[(size_t)(blo + ii) * n + blo + j] : 0.0;
+                        }
+                        __syncthreads();
+                        if (MSP_TID < 256) {
+#pragma unroll 8
+                            for (int k = 0; k < 32; ++k) {
+                                double a0 = As[(ty * 2) * 33 + k], a1 = As[(ty * 2 + 1) * 33 + k];
+                                double b0 = Bs[k * 33 + tx * 2], b1 = Bs[k * 33 + tx * 2 + 1];
+                                acc[0][0] = fma(a0, b0, acc[0][0]);
+                                acc[0][1] = fma(a0, b1, acc[0][1]);
+                                acc[1][0] = fma(a1, b0, acc[1][0]);
+                                acc[1][1] = fma(a1, b1, acc[1][1]);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (MSP_TID < 256) {
+                        for (int a = 0; a < 2; ++a)
+                            for (int b = 0; b < 2; ++b) {
+                                int r = r0 + ty * 2 + a, j = j0 + tx * 2 + b;
+                                if (r < sz && j < K)
+                                    Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc[a][b];
+                            }
+                    }
+#endif
+                }
+        }
+        lo = (bhi > lo) ? bhi : lo + 1;
+    }
+    MSP_SYNC();
 }
 
 // d (n): diagonal, overwritten by ascending eigenvalues; e (n-1): off-diagonal.
@@ -481,10 +567,13 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 }
                 W.outv[lo + r] = v;
                 W.outsrc[lo + r] = W.item_src[s];
+                if (W.item_src[s] >= 0) W.rootpos[lo + W.item_src[s]] = lo + r;
             }
         }
         MSP_SYNC();
         // --- new Q columns ---
+        // deflated columns and untouched blocks are copies; kept roots are
+        // Q[:, kcol] S gathered into QK and multiplied block by block.
         MSP_PFOR(idx, n * n)
         {
             int row = idx / n, col = idx % n;
@@ -492,22 +581,19 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
             dc_block(n, L, col, lo, hi);
             if (hi - lo >= 2) {
                 int src = W.outsrc[col];
-                double val = 0.0;
-                if (row >= lo && row < hi) {
-                    if (src >= 0) {
-                        int K = W.nkept[lo];
-                        for (int i = 0; i < K; ++i)
-                            val += Q[(size_t)row * n + W.kcol[lo + i]] * W.S[(size_t)(lo + i) * n + (lo + src)];
-                    } else {
-                        val = Q[(size_t)row * n + W.perm[~src]];
-                    }
-                }
-                Qn[(size_t)row * n + col] = val;
+                if (src < 0)
+                    Qn[(size_t)row * n + col] = (row >= lo && row < hi) ? Q[(size_t)row * n + W.perm[~src]] : 0.0;
+                else if (row < lo || row >= hi)
+                    Qn[(size_t)row * n + col] = 0.0;
+                int K = W.nkept[lo];
+                int i = col - lo;
+                if (i < K && row >= lo && row < hi) W.QK[(size_t)row * n + col] = Q[(size_t)row * n + W.kcol[col]];
             } else {
                 Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
             }
         }
         MSP_SYNC();
+        dc_merge_gemm(n, L, W, Qn, red);
         MSP_PFOR(s, n)
         {
             int lo, hi;
@@ -575,11 +661,19 @@ MSP_HD void backtransform_team(int n, const cdbl *A, const cdbl *tau, const doub
     for (int c0 = 0; c0 < n; c0 += cw) backtransform_chunk_team(n, A, tau, Z, U, Uc, cw, c0, wsum);
 }
 
+// Reflector block size of the blocked back-transformation.
+#define MSP_BT_NB 16
+// Panel width of the blocked tridiagonalisation.
+#define MSP_TD_NB 16
+
+MSP_HD int bt_nblocks(int n) { return (n - 1 + MSP_BT_NB - 1) / MSP_BT_NB; }
+
 // Doubles of per-matrix global workspace used by the divide-and-conquer path:
 // e (n), tau (2n), DcWork doubles (3n^2 + 12n) and ints (11n, padded).
 MSP_HD size_t dc_work_doubles(int n)
 {
-    return (size_t)3 * n * n + 15 * (size_t)n + 6 * (size_t)n + 8;
+    return (size_t)4 * n * n + 15 * (size_t)n + 7 * (size_t)n + 8 +
+           2 * (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB + 4 * (size_t)n * MSP_TD_NB;
 }
 
 // Carve a DcWork (and e, tau) out of one contiguous per-matrix block.
@@ -592,18 +686,290 @@ MSP_HD DcWork dc_carve(int n, double *base, double **e, cdbl **tau)
     W.Q0 = p; p += (size_t)n * n;
     W.Q1 = p; p += (size_t)n * n;
     W.S = p; p += (size_t)n * n;
+    W.QK = p; p += (size_t)n * n;
     W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
     W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
     W.rc = p; p += n; W.rs = p; p += n; W.item_v = p; p += n; W.outv = p; p += n;
     int *q = (int *)p;
     W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
     W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
-    W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n;
+    W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
     return W;
+}
+
+// T factors of the blocked back-transformation (after the DcWork arrays).
+MSP_HD cdbl *dc_tblocks(int n, double *base)
+{
+    return (cdbl *)(base + (size_t)4 * n * n + 15 * (size_t)n + 7 * (size_t)n + 8);
+}
+
+// Panel buffers Vp, Wp (n x MSP_TD_NB complex each) of the blocked reduction.
+MSP_HD cdbl *dc_panels(int n, double *base)
+{
+    return dc_tblocks(n, base) + (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB;
 }
 
 // The eigenvector matrix stedc_team leaves after dc_depth(n) buffer swaps.
 MSP_HD double *dc_result(int n, const DcWork &W)
 {
     return (dc_depth(n) & 1) ? W.Q1 : W.Q0;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked back-transformation.  Reflectors k0 .. k0+nb-1 combine into
+// H(k0) ... H(k0+nb-1) = I - V T V^H with V the (rows k0+1..n-1) x nb
+// unit-lower-trapezoidal matrix of reflector vectors and T upper triangular:
+//   T_jj = tau_j,  T(0:j, j) = -tau_j T(0:j, 0:j) (V(:, 0:j)^H v_j).
+// Tb (nb x nb complex, row-major) for block b is built by larft_block_team;
+// backtransform_blocked_team then applies U <- (I - V T V^H) U blockwise,
+// last block first, on an LDS column chunk of U.
+
+// V(r, j) of block k0 (absolute row r), from the reflector tails in A.
+MSP_HD cdbl vblock(int n, const cdbl *A, int k0, int r, int j)
+{
+    const int k = k0 + j;
+    if (r < k + 1) return cmk(0.0, 0.0);
+    if (r == k + 1) return cmk(1.0, 0.0);
+    return A[(size_t)r * n + k];
+}
+
+// T for reflectors [k0, k0 + nbk); G is team scratch of nb*nb complex.
+MSP_HD void larft_block_team(int n, const cdbl *A, const cdbl *tau, int k0, int nbk, int nb,
+                             cdbl *T, cdbl *G)
+{
+    // Gram G(i, j) = V(:, i)^H V(:, j), i < j
+    MSP_PFOR(idx, nbk * nbk)
+    {
+        int i = idx / nbk, j = idx % nbk;
+        cdbl acc = cmk(0.0, 0.0);
+        if (i < j) {
+            for (int r = k0 + j + 1; r < n; ++r)
+                cfmac(acc, vblock(n, A, k0, r, i), vblock(n, A, k0, r, j));
+        }
+        G[i * nb + j] = acc;
+    }
+    MSP_PFOR(idx, nb * nb) T[idx] = cmk(0.0, 0.0);
+    MSP_SYNC();
+    for (int j = 0; j < nbk; ++j) {
+        const cdbl tj = tau[k0 + j];
+        // T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j)
+        MSP_PFOR(i, j + 1)
+        {
+            if (i == j) {
+                T[j * nb + j] = tj;
+            } else {
+                cdbl acc = cmk(0.0, 0.0);
+                for (int l = i; l < j; ++l) cfma(acc, T[i * nb + l], G[l * nb + j]);
+                T[i * nb + j] = cmul(cmk(-tj.re, -tj.im), acc);
+            }
+        }
+        MSP_SYNC();
+    }
+}
+
+// Apply all blocks to the column chunk [c0, c0 + cw) of U = Q Z.
+// Uc: n x ldu complex (LDS), Tl: nb x nb, W, W2: nb x cw.
+MSP_HD void backtransform_blocked_team(int n, const cdbl *A, const cdbl *Tall, const double *Z,
+                                       cdbl *U, cdbl *Uc, int ldu, int cw, int c0, int nb,
+                                       cdbl *W, cdbl *W2)
+{
+    const int cwk = (n - c0 < cw) ? n - c0 : cw;
+    MSP_PFOR(idx, n * cwk)
+    {
+        int i = idx / cwk, c = idx % cwk;
+        Uc[i * ldu + c] = cmk(Z[(size_t)i * n + c0 + c], 0.0);
+    }
+    MSP_SYNC();
+    const int nref = n - 1;
+    const int nblk = (nref + nb - 1) / nb;
+    for (int b = nblk - 1; b >= 0; --b) {
+        const int k0 = b * nb;
+        const int nbk = (nref - k0 < nb) ? nref - k0 : nb;
+        const cdbl *T = Tall + (size_t)b * nb * nb;
+        // W = V^H Uc(k0+1 .., :)
+        MSP_PFOR(idx, nbk * cwk)
+        {
+            int j = idx / cwk, c = idx % cwk;
+            cdbl acc = cmk(0.0, 0.0);
+            for (int r = k0 + j + 1; r < n; ++r)
+                cfmac(acc, vblock(n, A, k0, r, j), Uc[r * ldu + c]);
+            W[j * cw + c] = acc;
+        }
+        MSP_SYNC();
+        // W2 = T W (T upper triangular)
+        MSP_PFOR(idx, nbk * cwk)
+        {
+            int j = idx / cwk, c = idx % cwk;
+            cdbl acc = cmk(0.0, 0.0);
+            for (int l = j; l < nbk; ++l) cfma(acc, T[j * nb + l], W[l * cw + c]);
+            W2[j * cw + c] = acc;
+        }
+        MSP_SYNC();
+        // Uc -= V W2
+        MSP_PFOR(idx, (n - k0 - 1) * cwk)
+        {
+            int r = k0 + 1 + idx / cwk, c = idx % cwk;
+            int jmax = r - k0 - 1;  // V(r, j) = 0 for j > r - k0 - 1
+            if (jmax > nbk - 1) jmax = nbk - 1;
+            cdbl acc = Uc[r * ldu + c];
+            for (int j = 0; j <= jmax; ++j) {
+                cdbl v = vblock(n, A, k0, r, j);
+                cdbl t = cmul(v, W2[j * cw + c]);
+                acc = csub(acc, t);
+            }
+            Uc[r * ldu + c] = acc;
+        }
+        MSP_SYNC();
+    }
+    MSP_PFOR(idx, n * cwk)
+    {
+        int i = idx / cwk, c = idx % cwk;
+        U[(size_t)i * n + c0 + c] = Uc[i * ldu + c];
+    }
+    MSP_SYNC();
+}
+
+// ---------------------------------------------------------------------------
+// Panel-blocked tridiagonalisation (same reduction as tridiag_team, with the
+// trailing-matrix update deferred to the end of each panel of nb columns).
+// Inside a panel, column k and the product A22 v use the panel-start matrix
+// corrected by the panel's earlier vectors:
+//   a(:, k) -= Vp W_p(k, :)^H + Wp Vp(k, :)^H,
+//   p = tau (A22 v - Vp (Wp^H v) - Wp (Vp^H v)),  w = p - (tau/2)(p^H v) v,
+// and after the panel the trailing block takes A -= Vp Wp^H + Wp Vp^H.
+// Vp, Wp: n x nb complex (row stride nb) per-matrix scratch (global memory);
+// y: 2*nb complex team scratch; vbuf, wbuf: n complex team scratch.
+MSP_HD void tridiag_blocked_team(int n, cdbl *A, double *d, double *e, cdbl *tau, int nb,
+                                 cdbl *Vp, cdbl *Wp, cdbl *vbuf, cdbl *wbuf, cdbl *y, double *red)
+{
+    for (int k0 = 0; k0 < n - 1; k0 += nb) {
+        const int nbk = (n - 1 - k0 < nb) ? n - 1 - k0 : nb;
+        MSP_PFOR(idx, n * nb)
+        {
+            Vp[idx] = cmk(0.0, 0.0);
+            Wp[idx] = cmk(0.0, 0.0);
+        }
+        MSP_SYNC();
+        for (int j = 0; j < nbk; ++j) {
+            const int k = k0 + j;
+            const int m = n - k - 1;
+            // 1. bring column k (rows k..n-1) up to date with the panel
+            if (j > 0) {
+                MSP_PFOR(ii, n - k)
+                {
+                    const int i = k + ii;
+                    cdbl a = A[(size_t)i * n + k];
+                    for (int l = 0; l < j; ++l) {
+                        a = csub(a, cmul(Vp[i * nb + l], cconj(Wp[k * nb + l])));
+                        a = csub(a, cmul(Wp[i * nb + l], cconj(Vp[k * nb + l])));
+                    }
+                    A[(size_t)i * n + k] = a;
+                }
+                MSP_SYNC();
+            }
+            cdbl *col = A + (size_t)(k + 1) * n + k;
+            // 2. reflector
+            double part = 0.0;
+            MSP_PFOR(i, m - 1) part += cabs2(col[(size_t)(i + 1) * n]);
+            const double xn2 = team_sum(part, red);
+            const cdbl alpha = col[0];
+            double beta;
+            cdbl tk, scale;
+            if (xn2 == 0.0 && alpha.im == 0.0) {
+                tk = cmk(0.0, 0.0);
+                beta = alpha.re;
+                scale = cmk(0.0, 0.0);
+            } else {
+                double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                beta = alpha.re >= 0.0 ? -nrm : nrm;
+                tk = cmk((beta - alpha.re) / beta, -alpha.im / beta);
+                scale = cdivc(cmk(1.0, 0.0), csub(alpha, cmk(beta, 0.0)));
+            }
+            MSP_PFOR(i, m)
+            {
+                cdbl vi;
+                if (i == 0) vi = cmk(1.0, 0.0);
+                else {
+                    vi = cmul(col[(size_t)i * n], scale);
+                    col[(size_t)i * n] = vi;
+                }
+                vbuf[i] = vi;
+                Vp[(k + 1 + i) * nb + j] = vi;
+            }
+            MSP_SYNC();
+            const bool act = (tk.re != 0.0 || tk.im != 0.0);
+            if (act) {
+                // y1 = Wp^H v, y2 = Vp^H v over rows k+1.. (earlier panel vectors)
+                MSP_PFOR(l, 2 * j)
+                {
+                    const cdbl *M = (l < j) ? Wp : Vp;
+                    const int c = (l < j) ? l : l - j;
+                    cdbl acc = cmk(0.0, 0.0);
+                    for (int i = 0; i < m; ++i) cfmac(acc, M[(k + 1 + i) * nb + c], vbuf[i]);
+                    y[l] = acc;
+                }
+                MSP_SYNC();
+                // p = tau (A22 v - Vp y1 - Wp y2)
+                const cdbl *A22 = A + (size_t)(k + 1) * n + (k + 1);
+                MSP_PFOR(i, m)
+                {
+                    cdbl acc = cmk(0.0, 0.0);
+                    for (int c = 0; c < m; ++c) cfmac(acc, A22[(size_t)c * n + i], vbuf[c]);
+                    for (int l = 0; l < j; ++l) {
+                        acc = csub(acc, cmul(Vp[(k + 1 + i) * nb + l], y[l]));
+                        acc = csub(acc, cmul(Wp[(k + 1 + i) * nb + l], y[j + l]));
+                    }
+                    wbuf[i] = cmul(tk, acc);
+                }
+                MSP_SYNC();
+                double pre = 0.0, pim = 0.0;
+                MSP_PFOR(i, m)
+                {
+                    cdbl t = cmulc(wbuf[i], vbuf[i]);
+                    pre += t.re;
+                    pim += t.im;
+                }
+                double sre = team_sum(pre, red);
+                double sim = team_sum(pim, red);
+                cdbl a2 = cmul(cmk(-0.5 * tk.re, -0.5 * tk.im), cmk(sre, sim));
+                MSP_PFOR(i, m) Wp[(k + 1 + i) * nb + j] = cadd(wbuf[i], cmul(a2, vbuf[i]));
+                MSP_SYNC();
+            } else {
+                MSP_PFOR(i, m) Vp[(k + 1 + i) * nb + j] = cmk(0.0, 0.0);
+                MSP_SYNC();
+            }
+#ifdef MSP_HOST_BUILD
+            d[k] = A[(size_t)k * n + k].re;
+            e[k] = beta;
+            tau[k] = tk;
+#else
+            if (MSP_TID == 0) {
+                d[k] = A[(size_t)k * n + k].re;
+                e[k] = beta;
+                tau[k] = tk;
+            }
+#endif
+            MSP_SYNC();
+        }
+        // trailing update A(t:, t:) -= Vp Wp^H + Wp Vp^H, t = k0 + nbk
+        const int t0 = k0 + nbk;
+        const int mt = n - t0;
+        MSP_PFOR(idx, mt * mt)
+        {
+            const int i = t0 + idx / mt, c = t0 + idx % mt;
+            cdbl a = A[(size_t)i * n + c];
+            for (int l = 0; l < nbk; ++l) {
+                a = csub(a, cmul(Vp[i * nb + l], cconj(Wp[c * nb + l])));
+                a = csub(a, cmul(Wp[i * nb + l], cconj(Vp[c * nb + l])));
+            }
+            A[(size_t)i * n + c] = a;
+        }
+        MSP_SYNC();
+    }
+#ifdef MSP_HOST_BUILD
+    d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+#else
+    if (MSP_TID == 0) d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+#endif
+    MSP_SYNC();
 }

@@ -1,0 +1,240 @@
+// Panel-blocked Householder tridiagonalisation that reads and writes only the
+// lower triangle (gfx950, 256 threads = 4 waves per matrix).
+//
+// Same reduction as tridiag_team in msp_tridiag.h (A = Q T Q^H, reflector
+// tails stored below the subdiagonal, d/e/tau out), organised for HBM:
+//   - the trailing update is deferred to the end of each panel of NB columns
+//     (zlatrd form): inside a panel, column k and the product A22 v use the
+//     panel-start matrix corrected by the panel's earlier vectors, kept in LDS;
+//   - A22 v is one pass over the lower triangle: each wave walks rows, lanes
+//     own columns; a row's left part is reduced across the wave (row sum)
+//     while conj(a_ic) v_i accumulates per lane (column sum), so every
+//     element is read once per step;
+//   - the end-of-panel update A -= Vp Wp^H + Wp Vp^H touches c <= i only.
+#include <hip/hip_runtime.h>
+
+#include "msp_kernels.h"
+#include "msp_tridiag.h"
+
+namespace {
+
+constexpr int kWaves = 4;
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NB, int QMAX>
+__global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E, double *work)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int LDP = NB + 1;                     // padded panel row stride
+    cdbl *Vp = (cdbl *)smem;                         // n x LDP
+    cdbl *Wp = Vp + (size_t)n * LDP;                 // n x LDP
+    cdbl *vb = Wp + (size_t)n * LDP;                 // n
+    cdbl *pb = vb + n;                               // n (row sums, then p)
+    cdbl *cp = pb + n;                               // kWaves x n column partials
+    cdbl *y = cp + (size_t)kWaves * n;               // 2 NB
+    double *red = (double *)(y + 2 * NB);            // 16
+
+    cdbl *A = (cdbl *)H + (size_t)b * n * n;
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    double *e;
+    cdbl *tau;
+    dc_carve(n, base, &e, &tau);
+    double *d = E + (size_t)b * n;
+
+    for (int k0 = 0; k0 < n - 1; k0 += NB) {
+        const int nbk = (n - 1 - k0 < NB) ? n - 1 - k0 : NB;
+        for (int idx = tid; idx < n * LDP; idx += 256) {
+            Vp[idx] = cmk(0.0, 0.0);
+            Wp[idx] = cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int j = 0; j < nbk; ++j) {
+            const int k = k0 + j;
+            const int m = n - k - 1;
+            // (a) column k, rows k..n-1, brought up to date within the panel
+            if (j > 0) {
+                for (int i = k + tid; i < n; i += 256) {
+                    cdbl a = A[(size_t)i * n + k];
+                    for (int l = 0; l < j; ++l) {
+                        a = csub(a, cmul(Vp[i * LDP + l], cconj(Wp[k * LDP + l])));
+                        a = csub(a, cmul(Wp[i * LDP + l], cconj(Vp[k * LDP + l])));
+                    }
+                    A[(size_t)i * n + k] = a;
+                }
+                __syncthreads();
+            }
+            // (b) reflector of A[k+1.., k]
+            double part = 0.0;
+            for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(A[(size_t)i * n + k]);
+            part = wave_sum(part);
+            if (lane == 0) red[wave] = part;
+            __syncthreads();
+            const double xn2 = red[0] + red[1] + red[2] + red[3];
+            const cdbl alpha = A[(size_t)(k + 1) * n + k];
+            double beta;
+            cdbl tk, scale;
+            if (xn2 == 0.0 && alpha.im == 0.0) {
+                tk = cmk(0.0, 0.0);
+                beta = alpha.re;
+                scale = cmk(0.0, 0.0);
+            } else {
+                double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+                beta = alpha.re >= 0.0 ? -nrm : nrm;
+                tk = cmk((beta - alpha.re) / beta, -alpha.im / beta);
+                scale = cdivc(cmk(1.0, 0.0), csub(alpha, cmk(beta, 0.0)));
+            }
+            const bool act = (tk.re != 0.0 || tk.im != 0.0);
+            for (int i = k + 1 + tid; i < n; i += 256) {
+                cdbl vi;
+                if (i == k + 1) vi = cmk(1.0, 0.0);
+                else {
+                    vi = cmul(A[(size_t)i * n + k], scale);
+                    A[(size_t)i * n + k] = vi;
+                }
+                vb[i] = vi;
+                Vp[i * LDP + j] = act ? vi : cmk(0.0, 0.0);
+            }
+            __syncthreads();
+            if (act) {
+                // (c) y[l] = Wp(:, l)^H v, y[j + l] = Vp(:, l)^H v over rows k+1..
+                for (int l = wave; l < 2 * j; l += kWaves) {
+                    const cdbl *M = (l < j) ? Wp : Vp;
+                    const int c = (l < j) ? l : l - j;
+                    double sr = 0.0, si = 0.0;
+                    for (int i = k + 1 + lane; i < n; i += 64) {
+                        cdbl t = cmulc(M[i * LDP + c], vb[i]);
+                        sr += t.re;
+                        si += t.im;
+                    }
+                    sr = wave_sum(sr);
+                    si = wave_sum(si);
+                    if (lane == 0) y[l] = cmk(sr, si);
+                }
+                // (d) A22 v over the lower triangle of rows/cols k+1..n-1
+                cdbl colacc[QMAX];
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) colacc[q] = cmk(0.0, 0.0);
+                for (int i = k + 1 + wave; i < n; i += kWaves) {
+                    const cdbl *row = A + (size_t)i * n;
+                    const cdbl vi = vb[i];
+                    double rr = 0.0, ri = 0.0;
+#pragma unroll
+                    for (int q = 0; q < QMAX; ++q) {
+                        const int c = k + 1 + lane + 64 * q;
+                        if (c <= i) {
+                            const cdbl a = row[c];
+                            const cdbl vc = vb[c];
+                            rr += a.re * vc.re - a.im * vc.im;
+                            ri += a.re * vc.im + a.im * vc.re;
+                            if (c < i) cfmac(colacc[q], a, vi);
+                        }
+                    }
+                    rr = wave_sum(rr);
+                    ri = wave_sum(ri);
+                    if (lane == 0) pb[i] = cmk(rr, ri);
+                }
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    const int c = k + 1 + lane + 64 * q;
+                    if (c < n) cp[wave * n + c] = colacc[q];
+                }
+                __syncthreads();
+                // (e) p = tau (A22 v - Vp y1 - Wp y2), and p^H v
+                double pre = 0.0, pim = 0.0;
+                for (int i = k + 1 + tid; i < n; i += 256) {
+                    cdbl acc = pb[i];
+                    for (int w = 0; w < kWaves; ++w) acc = cadd(acc, cp[w * n + i]);
+                    for (int l = 0; l < j; ++l) {
+                        acc = csub(acc, cmul(Vp[i * LDP + l], y[l]));
+                        acc = csub(acc, cmul(Wp[i * LDP + l], y[j + l]));
+                    }
+                    acc = cmul(tk, acc);
+                    pb[i] = acc;
+                    cdbl t = cmulc(acc, vb[i]);
+                    pre += t.re;
+                    pim += t.im;
+                }
+                pre = wave_sum(pre);
+                pim = wave_sum(pim);
+                __syncthreads();   // red reuse
+                if (lane == 0) {
+                    red[wave] = pre;
+                    red[4 + wave] = pim;
+                }
+                __syncthreads();
+                const double sre = red[0] + red[1] + red[2] + red[3];
+                const double sim = red[4] + red[5] + red[6] + red[7];
+                const cdbl a2 = cmul(cmk(-0.5 * tk.re, -0.5 * tk.im), cmk(sre, sim));
+                // (f) w = p + a2 v
+                for (int i = k + 1 + tid; i < n; i += 256) Wp[i * LDP + j] = cadd(pb[i], cmul(a2, vb[i]));
+            }
+            if (tid == 0) {
+                d[k] = A[(size_t)k * n + k].re;
+                e[k] = beta;
+                tau[k] = tk;
+            }
+            __syncthreads();
+            (void)m;
+        }
+        // end of panel: trailing lower triangle A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl)
+        const int t0 = k0 + nbk;
+        for (int i = t0 + wave; i < n; i += kWaves) {
+            cdbl vr[NB], wr[NB];
+#pragma unroll
+            for (int l = 0; l < NB; ++l) {
+                vr[l] = Vp[i * LDP + l];
+                wr[l] = Wp[i * LDP + l];
+            }
+            for (int c = t0 + lane; c <= i; c += 64) {
+                cdbl a = A[(size_t)i * n + c];
+#pragma unroll
+                for (int l = 0; l < NB; ++l) {
+                    const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
+                    a = csub(a, cmul(vr[l], cconj(wc)));
+                    a = csub(a, cmul(wr[l], cconj(vc)));
+                }
+                A[(size_t)i * n + c] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+}
+
+template <int NB, int QMAX>
+size_t lp_lds(int n)
+{
+    return sizeof(cdbl) * ((size_t)2 * n * (NB + 1) + 2 * (size_t)n + (size_t)kWaves * n + 2 * NB) +
+           16 * sizeof(double);
+}
+
+template <int NB, int QMAX>
+hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
+{
+    size_t lds = lp_lds<NB, QMAX>(n);
+    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX>), dim3(batch), dim3(256), lds, s, n, H, E, work);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Lower-triangle panel kernel for n <= 512; returns hipErrorNotSupported above.
+hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s)
+{
+    if (n <= 64) return lp_launch<8, 1>(n, batch, H, E, work, s);
+    if (n <= 128) return lp_launch<8, 2>(n, batch, H, E, work, s);
+    if (n <= 192) return lp_launch<8, 3>(n, batch, H, E, work, s);
+    if (n <= 256) return lp_launch<8, 4>(n, batch, H, E, work, s);
+    if (n <= 512) return lp_launch<4, 8>(n, batch, H, E, work, s);
+    return hipErrorNotSupported;
+}
