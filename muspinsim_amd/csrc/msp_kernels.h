@@ -44,3 +44,5 @@ hipError_t launch_accumulate(int nslots, int width, const int32_t *ptr, const in
 void prof_enable(int on);
 std::string prof_report();
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s);
+hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
+                                     hipStream_t s);

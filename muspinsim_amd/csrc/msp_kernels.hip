@@ -240,9 +240,7 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
     __shared__ cdbl T[MSP_BT_NB * MSP_BT_NB];
     const int b = blockIdx.y, blk = blockIdx.x;
     double *base = work + (size_t)b * dc_work_doubles(n);
-    double *e;
-    cdbl *tau;
-    dc_carve(n, base, &e, &tau);
+    const cdbl *tau = dc_tau_ptr(n, base);
     const int k0 = blk * MSP_BT_NB;
     const int nbk = (n - 1 - k0 < MSP_BT_NB) ? n - 1 - k0 : MSP_BT_NB;
     larft_block_team(n, (const cdbl *)H + (size_t)b * n * n, tau, k0, nbk, MSP_BT_NB, T, G);
@@ -730,6 +728,10 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
                            (const double *)H, work);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (n <= 512) {
+        ProfScope ps_("k_backtransform", s);
+        return launch_backtransform_mfma(n, batch, (const double *)H, work, U, s);
+    }
     const int cw = bt2_chunk_width(n);
     size_t lds3 = sizeof(cdbl) * ((size_t)n * (cw + 1) + 2 * MSP_BT_NB * cw);
     err = hipFuncSetAttribute((const void *)k_backtransform2,

@@ -709,6 +709,14 @@ MSP_HD cdbl *dc_panels(int n, double *base)
     return dc_tblocks(n, base) + (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB;
 }
 
+// Direct offsets into the per-matrix workspace (same layout as dc_carve), for
+// kernels that need only a few of the arrays.
+MSP_HD cdbl *dc_tau_ptr(int n, double *base) { return (cdbl *)(base + n); }
+MSP_HD double *dc_result_ptr(int n, double *base)
+{
+    return base + 3 * (size_t)n + ((dc_depth(n) & 1) ? (size_t)n * n : 0);
+}
+
 // The eigenvector matrix stedc_team leaves after dc_depth(n) buffer swaps.
 MSP_HD double *dc_result(int n, const DcWork &W)
 {
