@@ -75,7 +75,18 @@ def main():
                     help="instances of the sweep timed on the host (0 = skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy a device buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(instances shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)

@@ -43,7 +43,9 @@ struct DcAlloc {
     std::vector<double> dbl;
     std::vector<int> in;
     DcWork W;
-    DcAlloc(int n) : dbl((size_t)4 * n * n + 12 * (size_t)n), in(16 * (size_t)n)
+    // Workspace starts as NaN (doubles) / -1 (ints), as reused device memory
+    // would hold stale values: every read must follow a write.
+    DcAlloc(int n) : dbl((size_t)4 * n * n + 12 * (size_t)n, NAN), in(16 * (size_t)n, -1)
     {
         double *p = dbl.data();
         W.Q0 = p; p += (size_t)n * n;

@@ -224,7 +224,7 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 
 __global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info)
 {
-    __shared__ double red[2 * 32 * 33];
+    __shared__ double red[2 * 64 * 16];
     const int b = blockIdx.x;
     double *e;
     cdbl *tau;

@@ -71,6 +71,45 @@ MSP_HD void cfmac(cdbl &acc, cdbl a, cdbl b)
     acc.im = fma(-a.im, b.re, acc.im);
 }
 
+#ifndef MSP_HOST_BUILD
+// One DPP lane move of a double (two 32-bit halves); lanes outside ROWMASK
+// receive 0.  CTRL is a gfx9 DPP control: quad_perm, row mirrors or
+// row_bcast15/31.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_f64(double v)
+{
+    const long long x = __double_as_longlong(v);
+    int lo = (int)(x & 0xffffffffll), hi = (int)(x >> 32);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave on the VALU (no LDS traffic); the total is
+// exact in lane 63 only: pairs, quads, half rows, rows, then rows 0+1 and
+// 2+3 through row_bcast15 and the four rows through row_bcast31.
+__device__ inline double wave_sum_to63(double v)
+{
+    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);   // row_bcast15 -> rows 1, 3
+    v += dpp_f64<0x143, 0xC>(v);   // row_bcast31 -> rows 2, 3
+    return v;
+}
+
+// Wave sum broadcast to every lane.
+__device__ inline double wave_sum_all(double v)
+{
+    const double t = wave_sum_to63(v);
+    const long long x = __double_as_longlong(t);
+    const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+#endif
+
 // Team-wide sum of one double per thread.  `red` is team scratch of at least
 // MSP_NT doubles (LDS on the device).  Every thread returns the total.
 MSP_HD double team_sum(double v, double *red)
@@ -81,10 +120,10 @@ MSP_HD double team_sum(double v, double *red)
 #else
     const int tid = MSP_TID, nt = MSP_NT;
     // wave-level reduction first (64 lanes), then across waves through LDS
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = wave_sum_to63(v);
     const int wave = tid >> 6, nw = (nt + 63) >> 6;
     __syncthreads();
-    if ((tid & 63) == 0) red[wave] = v;
+    if ((tid & 63) == 63) red[wave] = v;
     __syncthreads();
     double t = 0.0;
     for (int w = 0; w < nw; ++w) t += red[w];

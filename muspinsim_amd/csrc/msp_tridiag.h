@@ -11,6 +11,9 @@
 // memory, the back-transformation works on LDS-resident column chunks.
 #pragma once
 #include "msp_team.h"
+#ifndef MSP_HOST_BUILD
+#include "msp_mfma.h"
+#endif
 
 // ---------------------------------------------------------------------------
 // (1) Householder tridiagonalisation, full Hermitian storage (row-major,
@@ -172,6 +175,14 @@ MSP_HD void dc_block(int n, int L, int i, int &lo, int &hi)
     }
 }
 
+// Largest block at tree level L (the rightmost path takes the ceiling half).
+MSP_HD int dc_maxblock(int n, int L)
+{
+    int lo, hi;
+    dc_block(n, L, n - 1, lo, hi);
+    return hi - lo;
+}
+
 MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho, int j,
                            int &org_out)
 {
@@ -246,13 +257,15 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
 // `red` scratch must hold 2 * 32 * 33 doubles); small ones are direct dots.
 MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red)
 {
-    // small blocks: one output per iteration
-    MSP_PFOR(idx, n * n)
+    // small blocks: one output per iteration, (column, row in its block) items
+    const int smax = dc_maxblock(n, L);
+    MSP_PFOR(idx, n * smax)
     {
-        int row = idx / n, col = idx % n;
+        int col = idx / smax;
         int lo, hi;
         dc_block(n, L, col, lo, hi);
-        if (hi - lo >= 2 && hi - lo < 48 && row >= lo && row < hi) {
+        int row = lo + idx % smax;
+        if (hi - lo >= 2 && hi - lo < 48 && row < hi) {
             int K = W.nkept[lo];
             int j = col - lo;
             if (j < K) {
@@ -263,63 +276,68 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
             }
         }
     }
-    // large blocks: tiled, one block after another
+    // large blocks, one after another
     for (int lo = 0; lo < n;) {
         int blo, bhi;
         dc_block(n, L, lo, blo, bhi);
         const int sz = bhi - blo;
         if (sz >= 48) {
             const int K = W.nkept[blo];
-            double *As = red;            // 32 x 33 (rows x k)
-            double *Bs = red + 32 * 33;  // 32 x 33 (k x cols)
-            for (int r0 = 0; r0 < sz; r0 += 32)
-                for (int j0 = 0; j0 < K; j0 += 32) {
 #ifdef MSP_HOST_BUILD
-                    for (int rr = 0; rr < 32; ++rr)
-                        for (int jj = 0; jj < 32; ++jj) {
-                            int r = r0 + rr, j = j0 + jj;
-                            if (r >= sz || j >= K) continue;
-                            double acc = 0.0;
-                            for (int i = 0; i < K; ++i)
-                                acc += W.QK[(size_t)(blo + r) * n + blo + i] * W.S[(size_t)(blo + i) * n + blo + j];
-                            Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
-                        }
-                    (void)As; (void)Bs;
-#else
-                    const int tx = MSP_TID % 16, ty = MSP_TID / 16;   // 256 threads
-                    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-                    for (int i0 = 0; i0 < K; i0 += 32) {
-                        for (int l = MSP_TID; l < 32 * 32; l += MSP_NT) {
-                            int a = l / 32, b = l % 32;
-                            int r = r0 + a, i = i0 + b;
-                            As[a * 33 + b] = (r < sz && i < K) ? W.QK[(size_t)(blo + r) * n + blo + i] : 0.0;
-                            int ii = i0 + a, j = j0 + b;
-                            Bs[a * 33 + b] = (ii < K && j < K) ? W.S[(size_t)(blo + ii) * n + blo + j] : 0.0;
-                        }
-                        __syncthreads();
-                        if (MSP_TID < 256) {
-#pragma unroll 8
-                            for (int k = 0; k < 32; ++k) {
-                                double a0 = As[(ty * 2) * 33 + k], a1 = As[(ty * 2 + 1) * 33 + k];
-                                double b0 = Bs[k * 33 + tx * 2], b1 = Bs[k * 33 + tx * 2 + 1];
-                                acc[0][0] = fma(a0, b0, acc[0][0]);
-                                acc[0][1] = fma(a0, b1, acc[0][1]);
-                                acc[1][0] = fma(a1, b0, acc[1][0]);
-                                acc[1][1] = fma(a1, b1, acc[1][1]);
-                            }
-                        }
-                        __syncthreads();
-                    }
-                    if (MSP_TID < 256) {
-                        for (int a = 0; a < 2; ++a)
-                            for (int b = 0; b < 2; ++b) {
-                                int r = r0 + ty * 2 + a, j = j0 + tx * 2 + b;
-                                if (r < sz && j < K)
-                                    Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc[a][b];
-                            }
-                    }
-#endif
+            (void)red;
+            for (int r = 0; r < sz; ++r)
+                for (int j = 0; j < K; ++j) {
+                    double acc = 0.0;
+                    for (int i = 0; i < K; ++i)
+                        acc += W.QK[(size_t)(blo + r) * n + blo + i] * W.S[(size_t)(blo + i) * n + blo + j];
+                    Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
                 }
+#else
+            // 64 x 64 output tiles on FP64 matrix cores: K panels of 16 of QK
+            // (64 x 16) and S (16 x 64) staged in LDS (`red`, 2048 doubles);
+            // each wave owns a 32 x 32 quadrant as 2 x 2 tiles of 16 x 16.
+            const int lane = MSP_TID & 63, wave = MSP_TID >> 6;
+            const int li = lane & 15, lq = lane >> 4;
+            double *Ap = red;              // [64][16]
+            double *Bp = red + 64 * 16;    // [16][64]
+            for (int rt = 0; rt < sz; rt += 64)
+                for (int ct = 0; ct < K; ct += 64) {
+                    msp_d4 acc[2][2];
+                    for (int x = 0; x < 2; ++x)
+                        for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
+                    for (int k0 = 0; k0 < K; k0 += 16) {
+                        for (int l = MSP_TID; l < 64 * 16; l += MSP_NT) {
+                            const int r = l / 16, kk = l % 16;     // A panel
+                            const int row = rt + r, col = k0 + kk;
+                            Ap[l] = (row < sz && col < K) ? W.QK[(size_t)(blo + row) * n + blo + col] : 0.0;
+                            const int kb = l / 64, cb = l % 64;    // B panel
+                            Bp[l] = (k0 + kb < K && ct + cb < K)
+                                        ? W.S[(size_t)(blo + k0 + kb) * n + blo + ct + cb] : 0.0;
+                        }
+                        __syncthreads();
+                        const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; ++s4) {
+                            const int kk = 4 * s4 + lq;
+                            const double a0 = Ap[(r0 + li) * 16 + kk], a1 = Ap[(r0 + 16 + li) * 16 + kk];
+                            const double b0 = Bp[kk * 64 + c0 + li], b1 = Bp[kk * 64 + c0 + 16 + li];
+                            acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+                            acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+                            acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+                            acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+                        }
+                        __syncthreads();
+                    }
+                    const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+                    for (int x = 0; x < 2; ++x)
+                        for (int y2 = 0; y2 < 2; ++y2)
+                            for (int q = 0; q < 4; ++q) {
+                                const int r = rt + r0 + 16 * x + 4 * q + lq;
+                                const int c = ct + c0 + 16 * y2 + li;
+                                if (r < sz && c < K) Qn[(size_t)(blo + r) * n + W.rootpos[blo + c]] = acc[x][y2][q];
+                            }
+                }
+#endif
         }
         lo = (bhi > lo) ? bhi : lo + 1;
     }
@@ -572,24 +590,34 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         }
         MSP_SYNC();
         // --- new Q columns ---
-        // deflated columns and untouched blocks are copies; kept roots are
-        // Q[:, kcol] S gathered into QK and multiplied block by block.
-        MSP_PFOR(idx, n * n)
+        // Q is block diagonal.  Each pass runs over (column, row of the
+        // column's parent block) items: rows inside the column's block get a
+        // copy (deflated columns, untouched blocks) or, for kept roots, the
+        // product Q[:, kcol] S (gathered into QK, multiplied below); rows of
+        // the parent block outside it are set to zero, because the next
+        // level's rotations and products read them.
         {
-            int row = idx / n, col = idx % n;
-            int lo, hi;
-            dc_block(n, L, col, lo, hi);
-            if (hi - lo >= 2) {
-                int src = W.outsrc[col];
-                if (src < 0)
-                    Qn[(size_t)row * n + col] = (row >= lo && row < hi) ? Q[(size_t)row * n + W.perm[~src]] : 0.0;
-                else if (row < lo || row >= hi)
-                    Qn[(size_t)row * n + col] = 0.0;
-                int K = W.nkept[lo];
-                int i = col - lo;
-                if (i < K && row >= lo && row < hi) W.QK[(size_t)row * n + col] = Q[(size_t)row * n + W.kcol[col]];
-            } else {
-                Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
+            const int pmax = (L > 0) ? dc_maxblock(n, L - 1) : n;
+            MSP_PFOR(idx, n * pmax)
+            {
+                const int col = idx / pmax;
+                int lo, hi, plo, phi;
+                dc_block(n, L, col, lo, hi);
+                if (L > 0) dc_block(n, L - 1, col, plo, phi);
+                else { plo = lo; phi = hi; }
+                const int row = plo + idx % pmax;
+                if (row < phi) {
+                    if (row < lo || row >= hi) {
+                        Qn[(size_t)row * n + col] = 0.0;
+                    } else if (hi - lo >= 2) {
+                        const int src = W.outsrc[col];
+                        if (src < 0) Qn[(size_t)row * n + col] = Q[(size_t)row * n + W.perm[~src]];
+                        if (col - lo < W.nkept[lo])
+                            W.QK[(size_t)row * n + col] = Q[(size_t)row * n + W.kcol[col]];
+                    } else {
+                        Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
+                    }
+                }
             }
         }
         MSP_SYNC();

@@ -20,11 +20,8 @@ namespace {
 
 constexpr int kWaves = 4;
 
-__device__ inline double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+// reductions across a wave run on DPP lane moves (msp_team.h)
+__device__ inline double wave_sum(double v) { return wave_sum_all(v); }
 
 template <int NB, int QMAX>
 __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E, double *work)
@@ -39,7 +36,9 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
     cdbl *pb = vb + n;                               // n (row sums, then p)
     cdbl *cp = pb + n;                               // kWaves x n column partials
     cdbl *y = cp + (size_t)kWaves * n;               // 2 NB
-    double *red = (double *)(y + 2 * NB);            // 16
+    cdbl *colA = y + 2 * NB;                         // n: current column k
+    cdbl *colB = colA + n;                           // n: column k+1, stashed by the matvec
+    double *red = (double *)(colB + n);              // 16
 
     cdbl *A = (cdbl *)H + (size_t)b * n * n;
     double *base = work + (size_t)b * dc_work_doubles(n);
@@ -48,6 +47,8 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
     dc_carve(n, base, &e, &tau);
     double *d = E + (size_t)b * n;
 
+    cdbl *col = colA, *nxt = colB;
+    for (int i = tid; i < n; i += 256) col[i] = A[(size_t)i * n];
     for (int k0 = 0; k0 < n - 1; k0 += NB) {
         const int nbk = (n - 1 - k0 < NB) ? n - 1 - k0 : NB;
         for (int idx = tid; idx < n * LDP; idx += 256) {
@@ -61,23 +62,23 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
             // (a) column k, rows k..n-1, brought up to date within the panel
             if (j > 0) {
                 for (int i = k + tid; i < n; i += 256) {
-                    cdbl a = A[(size_t)i * n + k];
+                    cdbl a = col[i];
                     for (int l = 0; l < j; ++l) {
                         a = csub(a, cmul(Vp[i * LDP + l], cconj(Wp[k * LDP + l])));
                         a = csub(a, cmul(Wp[i * LDP + l], cconj(Vp[k * LDP + l])));
                     }
-                    A[(size_t)i * n + k] = a;
+                    col[i] = a;
                 }
                 __syncthreads();
             }
             // (b) reflector of A[k+1.., k]
             double part = 0.0;
-            for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(A[(size_t)i * n + k]);
+            for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(col[i]);
             part = wave_sum(part);
             if (lane == 0) red[wave] = part;
             __syncthreads();
             const double xn2 = red[0] + red[1] + red[2] + red[3];
-            const cdbl alpha = A[(size_t)(k + 1) * n + k];
+            const cdbl alpha = col[k + 1];
             double beta;
             cdbl tk, scale;
             if (xn2 == 0.0 && alpha.im == 0.0) {
@@ -94,10 +95,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
             for (int i = k + 1 + tid; i < n; i += 256) {
                 cdbl vi;
                 if (i == k + 1) vi = cmk(1.0, 0.0);
-                else {
-                    vi = cmul(A[(size_t)i * n + k], scale);
-                    A[(size_t)i * n + k] = vi;
-                }
+                else vi = cmul(col[i], scale);
                 vb[i] = vi;
                 Vp[i * LDP + j] = act ? vi : cmk(0.0, 0.0);
             }
@@ -121,24 +119,56 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 cdbl colacc[QMAX];
 #pragma unroll
                 for (int q = 0; q < QMAX; ++q) colacc[q] = cmk(0.0, 0.0);
-                for (int i = k + 1 + wave; i < n; i += kWaves) {
-                    const cdbl *row = A + (size_t)i * n;
-                    const cdbl vi = vb[i];
-                    double rr = 0.0, ri = 0.0;
+                // four rows per pass (i, i+4, i+8, i+12 of this wave): every
+                // load of the pass is issued first, from clamped addresses with
+                // the value masked, so no branch separates the loads
+                for (int i0 = k + 1 + wave; i0 < n; i0 += 4 * kWaves) {
+                    cdbl av[4][QMAX];
 #pragma unroll
-                    for (int q = 0; q < QMAX; ++q) {
-                        const int c = k + 1 + lane + 64 * q;
-                        if (c <= i) {
-                            const cdbl a = row[c];
-                            const cdbl vc = vb[c];
-                            rr += a.re * vc.re - a.im * vc.im;
-                            ri += a.re * vc.im + a.im * vc.re;
-                            if (c < i) cfmac(colacc[q], a, vi);
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * kWaves;
+                        const int ic = (i < n) ? i : n - 1;
+                        const cdbl *row = A + (size_t)ic * n;
+#pragma unroll
+                        for (int q = 0; q < QMAX; ++q) {
+                            const int c = k + 1 + lane + 64 * q;
+                            const int cc = (c <= ic) ? c : ic;
+                            av[u][q] = row[cc];
                         }
                     }
-                    rr = wave_sum(rr);
-                    ri = wave_sum(ri);
-                    if (lane == 0) pb[i] = cmk(rr, ri);
+                    double rr[4], ri[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = i0 + u * kWaves;
+                        const bool rowok = i < n;
+                        const cdbl vi = vb[rowok ? i : n - 1];
+                        rr[u] = 0.0;
+                        ri[u] = 0.0;
+#pragma unroll
+                        for (int q = 0; q < QMAX; ++q) {
+                            const int c = k + 1 + lane + 64 * q;
+                            if (rowok && c <= i) {
+                                const cdbl a = av[u][q];
+                                if (q == 0 && lane == 0) nxt[i] = a;   // column k+1
+                                const cdbl vc = vb[c];
+                                rr[u] += a.re * vc.re - a.im * vc.im;
+                                ri[u] += a.re * vc.im + a.im * vc.re;
+                                if (c < i) cfmac(colacc[q], a, vi);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        rr[u] = wave_sum_to63(rr[u]);
+                        ri[u] = wave_sum_to63(ri[u]);
+                    }
+                    if (lane == 63) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int i = i0 + u * kWaves;
+                            if (i < n) pb[i] = cmk(rr[u], ri[u]);
+                        }
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < QMAX; ++q) {
@@ -175,43 +205,78 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 // (f) w = p + a2 v
                 for (int i = k + 1 + tid; i < n; i += 256) Wp[i * LDP + j] = cadd(pb[i], cmul(a2, vb[i]));
             }
+            if (!act)
+                for (int i = k + 1 + tid; i < n; i += 256) nxt[i] = A[(size_t)i * n + k + 1];
             if (tid == 0) {
-                d[k] = A[(size_t)k * n + k].re;
+                d[k] = col[k].re;
                 e[k] = beta;
                 tau[k] = tk;
             }
             __syncthreads();
+            {
+                cdbl *t = col;
+                col = nxt;
+                nxt = t;
+            }
             (void)m;
         }
-        // end of panel: trailing lower triangle A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl)
+        // end of panel: reflector tails of the panel's columns go to memory
+        // (rows k+2.., below the subdiagonal), and the trailing lower triangle
+        // takes A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl)
+        for (int idx = tid; idx < n * nbk; idx += 256) {
+            const int i = idx / nbk, l = idx % nbk;
+            if (i >= k0 + l + 2) A[(size_t)i * n + k0 + l] = Vp[i * LDP + l];
+        }
         const int t0 = k0 + nbk;
-        for (int i = t0 + wave; i < n; i += kWaves) {
-            cdbl vr[NB], wr[NB];
+        for (int i0 = t0 + wave; i0 < n; i0 += 2 * kWaves) {
+            cdbl av[2][QMAX];
 #pragma unroll
-            for (int l = 0; l < NB; ++l) {
-                vr[l] = Vp[i * LDP + l];
-                wr[l] = Wp[i * LDP + l];
+            for (int u = 0; u < 2; ++u) {
+                const int i = i0 + u * kWaves;
+                const int ic = (i < n) ? i : n - 1;
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    const int c = t0 + lane + 64 * q;
+                    const int cc = (c <= ic) ? c : ic;
+                    av[u][q] = A[(size_t)ic * n + cc];
+                }
             }
-            for (int c = t0 + lane; c <= i; c += 64) {
-                cdbl a = A[(size_t)i * n + c];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int i = i0 + u * kWaves;
+                if (i >= n) continue;
+                cdbl vr[NB], wr[NB];
 #pragma unroll
                 for (int l = 0; l < NB; ++l) {
-                    const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
-                    a = csub(a, cmul(vr[l], cconj(wc)));
-                    a = csub(a, cmul(wr[l], cconj(vc)));
+                    vr[l] = Vp[i * LDP + l];
+                    wr[l] = Wp[i * LDP + l];
                 }
-                A[(size_t)i * n + c] = a;
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    const int c = t0 + lane + 64 * q;
+                    if (c <= i) {
+                        cdbl a = av[u][q];
+#pragma unroll
+                        for (int l = 0; l < NB; ++l) {
+                            const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
+                            a = csub(a, cmul(vr[l], cconj(wc)));
+                            a = csub(a, cmul(wr[l], cconj(vc)));
+                        }
+                        A[(size_t)i * n + c] = a;
+                        if (c == t0) col[i] = a;                  // next step's column
+                    }
+                }
             }
         }
         __syncthreads();
     }
-    if (tid == 0) d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].re;
+    if (tid == 0) d[n - 1] = col[n - 1].re;
 }
 
 template <int NB, int QMAX>
 size_t lp_lds(int n)
 {
-    return sizeof(cdbl) * ((size_t)2 * n * (NB + 1) + 2 * (size_t)n + (size_t)kWaves * n + 2 * NB) +
+    return sizeof(cdbl) * ((size_t)2 * n * (NB + 1) + 4 * (size_t)n + (size_t)kWaves * n + 2 * NB) +
            16 * sizeof(double);
 }
 
