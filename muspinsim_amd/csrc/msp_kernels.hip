@@ -222,16 +222,34 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
     tridiag_team(n, (cdbl *)H + (size_t)b * n * n, E + (size_t)b * n, e, tau, vb, wb, red);
 }
 
-__global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info)
+// Divide and conquer on one tridiagonal per workgroup.  The n-sized arrays of
+// the merges live in LDS when `small_in_lds` (dynamic LDS holds them), the
+// n x n matrices (Q0, Q1, S, QK) in the global workspace.
+__global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info,
+                                               int small_in_lds)
 {
     __shared__ double red[2 * 64 * 16];
+    extern __shared__ __align__(16) unsigned char smem[];
     const int b = blockIdx.x;
     double *e;
     cdbl *tau;
     DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
+    if (small_in_lds) {
+        double *p = (double *)smem;
+        W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
+        W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
+        W.rc = p; p += n; W.rs = p; p += n; W.item_v = p; p += n; W.outv = p; p += n;
+        int *q = (int *)p;
+        W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
+        W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
+        W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
+        W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
+    }
     stedc_team(n, E + (size_t)b * n, e, W, red);
     if (threadIdx.x == 0) info[b] = 0;
 }
+
+size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 16; }
 
 // T factors of every reflector block: grid (blocks, instances).
 __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *work)
@@ -718,8 +736,14 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     }
     if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    { ProfScope ps_("k_stedc", s);
-    hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), 0, s, n, E, work, info);
+    {
+        const int in_lds = (n <= 512) ? 1 : 0;
+        const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
+        err = hipFuncSetAttribute((const void *)k_stedc, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds2);
+        if (err != hipSuccess) return err;
+        ProfScope ps_("k_stedc", s);
+        hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (bt_nblocks(n) > 0) {

@@ -155,6 +155,8 @@ struct DcWork {
     double *outv;              // merged eigenvalue per sorted output slot
     int *outsrc;               // item source of each output column
     int *rootpos;              // kept root j -> output column
+    int *blo, *bhi;            // block [lo, hi) of every position at the current level
+    int *plo, *phi;            // parent block (level L - 1; whole range at L = 0)
 };
 
 MSP_HD int dc_depth(int n)
@@ -259,12 +261,13 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
 {
     // small blocks: one output per iteration, (column, row in its block) items
     const int smax = dc_maxblock(n, L);
-    MSP_PFOR(idx, n * smax)
+    int sh = 0;
+    while ((1 << sh) < smax) ++sh;
+    MSP_PFOR(idx, n << sh)
     {
-        int col = idx / smax;
-        int lo, hi;
-        dc_block(n, L, col, lo, hi);
-        int row = lo + idx % smax;
+        const int col = idx >> sh;
+        const int lo = W.blo[col], hi = W.bhi[col];
+        const int row = lo + (idx & ((1 << sh) - 1));
         if (hi - lo >= 2 && hi - lo < 48 && row < hi) {
             int K = W.nkept[lo];
             int j = col - lo;
@@ -278,8 +281,7 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
     }
     // large blocks, one after another
     for (int lo = 0; lo < n;) {
-        int blo, bhi;
-        dc_block(n, L, lo, blo, bhi);
+        const int blo = W.blo[lo], bhi = W.bhi[lo];
         const int sz = bhi - blo;
         if (sz >= 48) {
             const int K = W.nkept[blo];
@@ -369,11 +371,22 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
 
     const int depth = dc_depth(n);
     for (int L = depth - 1; L >= 0; --L) {
-        // --- z and sorted order per block (by position) ---
+        // block table of this level and its parent level
         MSP_PFOR(i, n)
         {
             int lo, hi;
             dc_block(n, L, i, lo, hi);
+            W.blo[i] = lo;
+            W.bhi[i] = hi;
+            if (L > 0) dc_block(n, L - 1, i, lo, hi);
+            W.plo[i] = lo;
+            W.phi[i] = hi;
+        }
+        MSP_SYNC();
+        // --- z and sorted order per block (by position) ---
+        MSP_PFOR(i, n)
+        {
+            const int lo = W.blo[i], hi = W.bhi[i];
             if (hi - lo >= 2) {
                 int mid = lo + (hi - lo) / 2;
                 double ec = e[mid - 1] * inv;
@@ -399,8 +412,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         MSP_SYNC();
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 W.ds[s] = W.dv[W.perm[s]];
                 W.zs[s] = W.zv[W.perm[s]];
@@ -410,8 +422,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- deflation scan, one thread per block ---
         MSP_PFOR(lo0, n)
         {
-            int lo, hi;
-            dc_block(n, L, lo0, lo, hi);
+            const int lo = W.blo[lo0], hi = W.bhi[lo0];
             if (lo == lo0 && hi - lo >= 2) {
                 int mid = lo + (hi - lo) / 2;
                 double rho = 2.0 * fabs(e[mid - 1] * inv);
@@ -471,8 +482,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- apply the deflation rotations to Q rows (in scan order) ---
         MSP_PFOR(row, n)
         {
-            int lo, hi;
-            dc_block(n, L, row, lo, hi);
+            const int lo = W.blo[row], hi = W.bhi[row];
             if (hi - lo >= 2) {
                 int nr = W.nrot[lo];
                 for (int r = 0; r < nr; ++r) {
@@ -487,8 +497,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- secular roots ---
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 int mid = lo + (hi - lo) / 2;
                 double rho = 2.0 * fabs(e[mid - 1] * inv);
@@ -506,8 +515,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- Loewner z ---
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 int K = W.nkept[lo];
                 int i = s - lo;
@@ -530,8 +538,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- eigenvectors of the rank-one problem: S[i][j] (block-local square) ---
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 int K = W.nkept[lo];
                 int j = s - lo;
@@ -555,8 +562,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- merge items (kept roots, then deflated slots) ---
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 int K = W.nkept[lo];
                 int t = s - lo;
@@ -574,8 +580,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // --- ascending order of the merged eigenvalues, ties by item index ---
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) {
                 double v = W.item_v[s];
                 int r = 0;
@@ -598,14 +603,14 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // level's rotations and products read them.
         {
             const int pmax = (L > 0) ? dc_maxblock(n, L - 1) : n;
-            MSP_PFOR(idx, n * pmax)
+            int sh = 0;
+            while ((1 << sh) < pmax) ++sh;
+            MSP_PFOR(idx, n << sh)
             {
-                const int col = idx / pmax;
-                int lo, hi, plo, phi;
-                dc_block(n, L, col, lo, hi);
-                if (L > 0) dc_block(n, L - 1, col, plo, phi);
-                else { plo = lo; phi = hi; }
-                const int row = plo + idx % pmax;
+                const int col = idx >> sh;
+                const int lo = W.blo[col], hi = W.bhi[col];
+                const int plo = W.plo[col], phi = W.phi[col];
+                const int row = plo + (idx & ((1 << sh) - 1));
                 if (row < phi) {
                     if (row < lo || row >= hi) {
                         Qn[(size_t)row * n + col] = 0.0;
@@ -624,8 +629,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         dc_merge_gemm(n, L, W, Qn, red);
         MSP_PFOR(s, n)
         {
-            int lo, hi;
-            dc_block(n, L, s, lo, hi);
+            const int lo = W.blo[s], hi = W.bhi[s];
             if (hi - lo >= 2) W.dv[s] = W.outv[s];
         }
         MSP_SYNC();
@@ -700,7 +704,7 @@ MSP_HD int bt_nblocks(int n) { return (n - 1 + MSP_BT_NB - 1) / MSP_BT_NB; }
 // e (n), tau (2n), DcWork doubles (3n^2 + 12n) and ints (11n, padded).
 MSP_HD size_t dc_work_doubles(int n)
 {
-    return (size_t)4 * n * n + 15 * (size_t)n + 7 * (size_t)n + 8 +
+    return (size_t)4 * n * n + 15 * (size_t)n + 9 * (size_t)n + 8 +
            2 * (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB + 4 * (size_t)n * MSP_TD_NB;
 }
 
@@ -722,13 +726,14 @@ MSP_HD DcWork dc_carve(int n, double *base, double **e, cdbl **tau)
     W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
     W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
     W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
+    W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
     return W;
 }
 
 // T factors of the blocked back-transformation (after the DcWork arrays).
 MSP_HD cdbl *dc_tblocks(int n, double *base)
 {
-    return (cdbl *)(base + (size_t)4 * n * n + 15 * (size_t)n + 7 * (size_t)n + 8);
+    return (cdbl *)(base + (size_t)4 * n * n + 15 * (size_t)n + 9 * (size_t)n + 8);
 }
 
 // Panel buffers Vp, Wp (n x MSP_TD_NB complex each) of the blocked reduction.
