@@ -38,7 +38,8 @@ def algorithmic(kernel, n, ntimes, real_mode=True):
     if kernel == "k_eigh_jacobi":
         return "flop", (16.0 / 3.0 + 8.0) * n ** 3
     if kernel == "k_stedc":
-        return "flop", 2.0 * n ** 3
+        # the merge products Q_blockdiag Z: n^3 at the top level, n^3/4 below, ...
+        return "flop", 4.0 / 3.0 * n ** 3
     if kernel == "k_timesum":
         return "flop", (4.0 if real_mode else 8.0) * n * n * ntimes
     if kernel == "k_integral":
@@ -52,16 +53,30 @@ def algorithmic(kernel, n, ntimes, real_mode=True):
     return "byte", 0.0
 
 
+def measured_traffic(tag, kernel):
+    """HBM bytes per instance of `kernel` from the committed counter summary
+    (FETCH_SIZE / WRITE_SIZE scaled by a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{tag}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    for name, rec in data["kernels"].items():
+        if name == kernel or name.startswith(kernel + "_") or name.startswith(kernel + "<"):
+            return rec["fetch_bytes_per_instance"] + rec["write_bytes_per_instance"]
+    return None
+
+
 def workload(name):
     from muspinsim_amd import workloads as W
     if name == "c2":
-        return W.c2_alc_sweep(), "alc_sweep_n192_200fields_x_500orient_integral"
+        return W.c2_alc_sweep(), "alc_sweep_n192_200fields_x_500orient_integral", "alc_sweep_n192"
     if name == "c1":
-        return W.c1_zero_field_powder(), "zero_field_powder_n16_100orient_1000t"
+        return W.c1_zero_field_powder(), "zero_field_powder_n16_100orient_1000t", "zero_field_n16"
     if name == "c3":
-        return W.c3_tf_stress(), "tf_stress_n512_256orient_4000t"
+        return W.c3_tf_stress(), "tf_stress_n512_256orient_4000t", "tf_stress_n512"
     if name == "c5":
-        return W.c5_fit(), "fit_forward_n64_256orient_2000t"
+        return W.c5_fit(), "fit_forward_n64_256orient_2000t", "fit_forward_n64"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -71,7 +86,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-sample", type=int, default=4000,
+    ap.add_argument("--cpu-sample", type=int, default=24000,
                     help="instances of the sweep timed on the host (0 = skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
@@ -93,7 +108,7 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    (systems, batch), wname = workload(args.workload)
+    (systems, batch), wname, tag = workload(args.workload)
     n = systems[0].n
     ninst = len(batch)
     width = 1 if batch.y == "integral" else len(batch.times)
@@ -166,6 +181,10 @@ def main():
         achieved = per_inst * inst_per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    per_inst_bytes = measured_traffic(tag, kname)
+    if per_inst_bytes is not None:
+        roof["traffic"] = per_inst_bytes * inst_per_launch
+        roof["traffic_source"] = f"profiles/{tag}_hbm_traffic.json"
     roof["kernel"] = kname
     roof["avg_launch_ms"] = tot_ms / launches
     roof["instances_per_launch"] = inst_per_launch
