@@ -31,6 +31,7 @@
  *   msp_timesum         evolve time loop, hamiltonian.py:87-107, and the
  *                       cosine sum parallel_fast_time_evolve, cython/parallel.pyx:16-69
  *   msp_integral        Hamiltonian.integrate_decaying, hamiltonian.py:150-162
+ *   msp_phase_evolve    evolve without operators (density matrices), hamiltonian.py:108-116
  *   msp_accumulate      Re(data)*w and store_time_slice powder accumulation,
  *                       experiment.py:498; simconfig.py:347-368
  */
@@ -137,6 +138,12 @@ int msp_timesum(int n, int batch, int ntimes, const double *d_E, const double *d
 int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode,
                  double tau, double out_scale, const double *d_w, int out_complex,
                  double *d_out, void *stream);
+
+/* d_out[t] = exp(-2 pi i (E_i - E_j) t) rho_ij for one instance (n x n complex
+ * per time, ntimes <= 65535): the eigenbasis states that Hamiltonian.evolve
+ * returns when no operator is given (muspinsim/hamiltonian.py:108-116). */
+int msp_phase_evolve(int n, int ntimes, const double *d_E, const double *d_rho,
+                     const double *d_times, double *d_out, void *stream);
 
 /* results[s][t] += rows[i][t] * scale for i in d_idx[d_ptr[s] .. d_ptr[s+1]),
  * in that order (deterministic powder accumulation). */

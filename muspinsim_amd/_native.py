@@ -56,6 +56,7 @@ SIGNATURES = {
     "msp_form_c": (_I, [_I, _I, _P, _P, _I, _D, _D, _P, _P]),
     "msp_timesum": (_I, [_I, _I, _I, _P, _P, _I, _P, _P, _I, _P, _P]),
     "msp_integral": (_I, [_I, _I, _P, _P, _I, _D, _D, _P, _I, _P, _P]),
+    "msp_phase_evolve": (_I, [_I, _I, _P, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
     "msp_profile_enable": (_I, [_I]),
     "msp_profile_report": (_I, [ctypes.c_char_p, _SZ]),

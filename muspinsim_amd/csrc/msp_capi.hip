@@ -211,6 +211,19 @@ int msp_integral(int n, int batch, const double *d_E, const double *d_C, int rea
                       "msp_integral");
 }
 
+int msp_phase_evolve(int n, int ntimes, const double *d_E, const double *d_rho,
+                     const double *d_times, double *d_out, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_phase_evolve: n out of range");
+    if (ntimes < 0 || ntimes > 65535)
+        return fail(MSP_ERR_VALUE, "msp_phase_evolve: ntimes = %d out of range", ntimes);
+    if (ntimes == 0) return MSP_OK;
+    if (!d_E || !d_rho || !d_times || !d_out)
+        return fail(MSP_ERR_VALUE, "msp_phase_evolve: null pointer");
+    return hip_status(launch_phase_evolve(n, ntimes, d_E, d_rho, d_times, d_out, S(stream)),
+                      "msp_phase_evolve");
+}
+
 int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
                    const double *d_rows, double scale, double *d_results, void *stream)
 {

@@ -35,6 +35,8 @@ hipError_t launch_form_c(int n, int batch, const double *rho, const double *obs,
 hipError_t launch_timesum(int n, int batch, int ntimes, const double *E, const double *C,
                           int real_mode, const double *times, const double *w, int out_complex,
                           double *out, hipStream_t s);
+hipError_t launch_phase_evolve(int n, int ntimes, const double *E, const double *rho,
+                              const double *times, double *out, hipStream_t s);
 hipError_t launch_integral(int n, int batch, const double *E, const double *C, int real_mode,
                            double tau, double out_scale, const double *w, int out_complex,
                            double *out, hipStream_t s);

@@ -62,15 +62,9 @@ __global__ void k_rotate(int batch, const double *q, const double *B, const doub
 
 // ---------------------------------------------------------------------------
 // H[b][a][c]: terms (or a precomputed base) + Zeeman, one thread per element.
-__global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
-                          const int32_t *sys_id, const double *Br, const double *base,
-                          int zeeman, double *H)
+__device__ cdbl h_element(int n, const MspSystem &S, const MspTerm *terms, const cdbl *bm,
+                          const double *B, int zeeman, int idx)
 {
-    const int b = blockIdx.y;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * n) return;
-    const int s = sys_id ? sys_id[b] : 0;
-    const MspSystem &S = systems[s];
     const int ar = idx / n, ac = idx % n;
     int dr[MSP_MAX_SPINS], dc[MSP_MAX_SPINS];
     int ndiff = 0, p0 = -1, p1 = -1;
@@ -89,12 +83,9 @@ __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
         }
     }
     cdbl acc = cmk(0.0, 0.0);
-    if (base) {
-        const cdbl *bm = (const cdbl *)base + (size_t)s * n * n;
-        acc = bm[idx];
-    }
+    if (bm) acc = bm[idx];
     if (ndiff <= 2) {
-        if (!base) {
+        if (!bm) {
             for (int t = 0; t < S.n_terms; ++t) {
                 const MspTerm &T = terms[S.term_offset + t];
                 if (T.kind == 0) {
@@ -134,7 +125,6 @@ __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
             }
         }
         if (zeeman && ndiff <= 1) {
-            const double *B = Br + 3 * b;
             for (int i = 0; i < S.n_spins; ++i) {
                 if (ndiff == 1 && p0 != i) continue;
                 double g = S.gammas[i];
@@ -143,7 +133,87 @@ __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
             }
         }
     }
-    ((cdbl *)H)[(size_t)b * n * n + idx] = acc;
+    return acc;
+}
+
+// H[b][a][c]: the system's terms (or a precomputed base) + Zeeman, one thread
+// per element, index digits by division (any multiplicities).
+__global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
+                          const int32_t *sys_id, const double *Br, const double *base,
+                          int zeeman, double *H)
+{
+    const int b = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int s = sys_id ? sys_id[b] : 0;
+    const cdbl *bm = base ? (const cdbl *)base + (size_t)s * n * n : nullptr;
+    ((cdbl *)H)[(size_t)b * n * n + idx] = h_element(n, systems[s], terms, bm, Br + 3 * b, zeeman, idx);
+}
+
+// H[b] = base[sys[b]] + sum_i gamma_i Br[b].S_i.  The spin digits of every
+// basis index are packed 4 bits per spin into one LDS word per index, so an
+// element needs one XOR: zero means diagonal (only S_z contributes), a single
+// non-zero nibble whose digits differ by one means a one-spin flip (S_x, S_y),
+// anything else is the base element alone.  Systems with a multiplicity above
+// 16 take the per-element path.
+constexpr int kBuildEPT = 4;
+__global__ void __launch_bounds__(256) k_build_h_base(int n, const MspSystem *systems,
+                                                      const MspTerm *terms, const int32_t *sys_id,
+                                                      const double *Br, const double *base, double *H)
+{
+    extern __shared__ unsigned long long digs[];
+    const int b = blockIdx.y;
+    const int s = sys_id ? sys_id[b] : 0;
+    const MspSystem &S = systems[s];
+    const int ns = S.n_spins;
+    bool packable = true;
+    for (int k = 0; k < ns; ++k) packable = packable && (S.dims[k] <= 16);
+    const cdbl *bm = (const cdbl *)base + (size_t)s * n * n;
+    cdbl *Hb = (cdbl *)H + (size_t)b * n * n;
+    const double *B = Br + 3 * b;
+    const int nn = n * n;
+    const int idx0 = blockIdx.x * 256 * kBuildEPT + threadIdx.x;
+    if (!packable) {
+        for (int e = 0; e < kBuildEPT; ++e) {
+            const int idx = idx0 + e * 256;
+            if (idx < nn) Hb[idx] = h_element(n, S, terms, bm, B, 1, idx);
+        }
+        return;
+    }
+    for (int r = threadIdx.x; r < n; r += 256) {
+        unsigned long long w = 0;
+        int x = r;
+        for (int k = ns - 1; k >= 0; --k) {
+            const int d = S.dims[k];
+            w |= (unsigned long long)(x % d) << (4 * k);
+            x /= d;
+        }
+        digs[r] = w;
+    }
+    __syncthreads();
+    for (int e = 0; e < kBuildEPT; ++e) {
+        const int idx = idx0 + e * 256;
+        if (idx >= nn) break;
+        const int r = idx / n, c = idx - r * n;
+        cdbl acc = bm[idx];
+        const unsigned long long dr = digs[r], x = dr ^ digs[c];
+        if (x == 0) {
+            double z = 0.0;
+            for (int k = 0; k < ns; ++k)
+                z += S.gammas[k] * (0.5 * (S.dims[k] - 1) - (double)((dr >> (4 * k)) & 15));
+            acc.re += B[2] * z;
+        } else {
+            const int k = __builtin_ctzll(x) >> 2;
+            if ((x >> (4 * k)) <= 15) {
+                const int a = (int)((dr >> (4 * k)) & 15), c2 = (int)((digs[c] >> (4 * k)) & 15);
+                const int d = S.dims[k];
+                const double g = S.gammas[k];
+                acc = cadd(acc, cscale(spin_el(d, 0, a, c2), g * B[0]));
+                acc = cadd(acc, cscale(spin_el(d, 1, a, c2), g * B[1]));
+            }
+        }
+        Hb[idx] = acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -524,6 +594,22 @@ template __global__ void k_timesum<false>(int, int, const double *, const double
                                           const double *, int, double *);
 
 // ---------------------------------------------------------------------------
+// out[t]_ij = exp(-2 pi i (E_i - E_j) t) rho_ij: the eigenbasis density matrix
+// of one instance at every time.
+__global__ void k_phase_evolve(int n, int ntimes, const double *E, const double *rho,
+                               const double *times, double *out)
+{
+    const int t = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n || t >= ntimes) return;
+    const int i = idx / n, j = idx - (idx / n) * n;
+    double sn, cs;
+    sincos(-6.283185307179586 * (E[i] - E[j]) * times[t], &sn, &cs);
+    const cdbl r = ((const cdbl *)rho)[idx];
+    ((cdbl *)out)[(size_t)t * n * n + idx] = cmul(cmk(cs, sn), r);
+}
+
+// ---------------------------------------------------------------------------
 // I = sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)), one workgroup per instance.
 __global__ void __launch_bounds__(256) k_integral(int n, const double *E, const double *C,
                                                   int real_mode, double tau, double out_scale,
@@ -693,9 +779,14 @@ hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspT
                           const int32_t *sys_id, const double *Br, const double *base, int zeeman,
                           double *H, hipStream_t s)
 {
-    { ProfScope ps_("k_build_h", s);
-    hipLaunchKernelGGL(k_build_h, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
-                       systems, terms, sys_id, Br, base, zeeman, H);
+    ProfScope ps_("k_build_h", s);
+    if (base && zeeman) {
+        hipLaunchKernelGGL(k_build_h_base, dim3(cdiv((size_t)n * n, 256 * kBuildEPT), batch),
+                           dim3(256), sizeof(unsigned long long) * n, s, n, systems, terms, sys_id,
+                           Br, base, H);
+    } else {
+        hipLaunchKernelGGL(k_build_h, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
+                           systems, terms, sys_id, Br, base, zeeman, H);
     }
     return hipGetLastError();
 }
@@ -823,6 +914,15 @@ hipError_t launch_timesum(int n, int batch, int ntimes, const double *E, const d
         hipLaunchKernelGGL(k_timesum<false>, grid, dim3(256), 0, s, n, ntimes, E, C, times, w,
                            out_complex, out);
         }
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_evolve(int n, int ntimes, const double *E, const double *rho,
+                              const double *times, double *out, hipStream_t s)
+{
+    ProfScope ps_("k_phase_evolve", s);
+    hipLaunchKernelGGL(k_phase_evolve, dim3(cdiv((size_t)n * n, 256), ntimes), dim3(256), 0, s, n,
+                       ntimes, E, rho, times, out);
     return hipGetLastError();
 }
 

@@ -117,7 +117,7 @@ class SpinEngine:
     """Runs instance batches through the HIP pipeline on one device."""
 
     def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=6 << 30,
-                 max_chunk=8192):
+                 max_chunk=8192, base_bytes=1 << 30):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("SpinEngine needs a HIP device (torch 'cuda' device on ROCm)")
@@ -125,6 +125,7 @@ class SpinEngine:
         self.method = method
         self.chunk_bytes = chunk_bytes
         self.max_chunk = max_chunk
+        self.base_bytes = base_bytes
 
     # ------------------------------------------------------------------ utils
     def stream(self):
@@ -199,6 +200,15 @@ class SpinEngine:
         N.check(self.lib.msp_integral(n, B, E.data_ptr(), C.data_ptr(), int(real_mode), float(tau),
                                       float(out_scale), None if w is None else w.data_ptr(),
                                       int(out_complex), out.data_ptr(), self.stream()))
+        return out
+
+    def phase_evolve(self, E, rho, times):
+        """(T, n, n): exp(-2 pi i (E_i - E_j) t) rho_ij for one instance."""
+        n = E.shape[-1]
+        T = times.shape[0]
+        out = torch.empty((T, n, n), dtype=torch.complex128, device=self.device)
+        N.check(self.lib.msp_phase_evolve(n, T, E.data_ptr(), rho.data_ptr(), times.data_ptr(),
+                                          out.data_ptr(), self.stream()))
         return out
 
     def apply_kron(self, dims, F, U):
@@ -289,6 +299,15 @@ class SpinEngine:
                 if route == "fast":
                     s0 = systems[int(batch.sys_id[idx[0]])]
                     g.d_other = n // s0.dims[s0.muon_index]
+                # systems shared by many instances get their field-independent
+                # H_sys built once per execute; H = H_sys + Zeeman per instance
+                uniq, local = np.unique(batch.sys_id[idx], return_inverse=True)
+                g.base = None
+                if idx.size >= 2 * uniq.size and uniq.size * n * n * 16 <= self.base_bytes:
+                    g.base = torch.empty((uniq.size, n, n), dtype=torch.complex128, device=self.device)
+                    g.base_sys = self._bytes_view(sysarr[uniq])
+                    g.base_ids = self._dev(np.arange(uniq.size, dtype=np.int32), torch.int32)
+                    g.sid_local = self._dev(local.astype(np.int32), torch.int32)
                 self._alloc(g)
                 plan.groups.append(g)
         order = np.argsort(batch.slot, kind="stable")
@@ -324,6 +343,10 @@ class SpinEngine:
         plan.results.zero_()
         for g in plan.groups:
             n = g.n
+            if g.base is not None:
+                N.check(L.msp_build_h(n, g.base.shape[0], g.base_sys.data_ptr(),
+                                      plan.d_terms.data_ptr(), g.base_ids.data_ptr(), None, None, 0,
+                                      g.base.data_ptr(), st))
             for c0 in range(0, g.idx.size, g.cs):
                 nb = min(g.cs, g.idx.size - c0)
                 sl = slice(c0, c0 + nb)
@@ -333,8 +356,13 @@ class SpinEngine:
 
                 N.check(L.msp_rotate_fields(nb, p(g.q), p(g.B), p(g.Bint), p(g.p),
                                             g.Br.data_ptr(), g.pr.data_ptr(), st))
-                N.check(L.msp_build_h(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
-                                      p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
+                if g.base is not None:
+                    N.check(L.msp_build_h(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
+                                          p(g.sid_local), g.Br.data_ptr(), g.base.data_ptr(), 1,
+                                          g.H.data_ptr(), st))
+                else:
+                    N.check(L.msp_build_h(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
+                                          p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
                 N.check(L.msp_eigh(n, nb, g.H.data_ptr(), g.E.data_ptr(), g.U.data_ptr(),
                                    g.info.data_ptr(), self.method, g.work.data_ptr(), g.wsb, st))
                 Y = g.H   # destroyed by the eigensolver; reused as scratch

@@ -101,6 +101,52 @@ def test_build_h_matches_oracle(eng):
             assert np.abs(H[k] - Href).max() <= 1e-12 * np.abs(Href).max()
 
 
+def _rand_system(rng, Is):
+    from muspinsim_amd.engine import SystemDesc
+    terms = [("linear", 0, 0, rng.normal(size=3))]
+    for i in range(len(Is)):
+        for j in range(i, len(Is)):
+            if i == j and Is[i] < 1:
+                continue
+            M = rng.normal(size=(3, 3)) * 10
+            terms.append(("bilinear", i, j, (M + M.T) / 2 if i == j else M))
+    return SystemDesc(Is=list(Is), gammas=list(rng.normal(size=len(Is)) * 100), terms=terms)
+
+
+@pytest.mark.parametrize("Is", [[0.5, 0.5, 1.0], [0.5, 3.5, 0.5], [0.5, 8.5]])
+def test_build_h_from_base(eng, Is):
+    """H = H_sys (built once, zeeman = 0) + Zeeman per instance equals the
+    direct build; d = 18 (I = 17/2) exercises the per-element fallback."""
+    from muspinsim_amd import _native as N
+    from muspinsim_amd.engine import pack_systems
+    L = N.lib()
+    rng = np.random.default_rng(len(Is) + int(2 * max(Is)))
+    systems = [_rand_system(rng, Is), _rand_system(rng, Is)]
+    n = systems[0].n
+    sysarr, termarr = pack_systems(systems)
+    dsys, dterms = eng._bytes_view(sysarr), eng._bytes_view(termarr)
+    nb = 5
+    sid = _dev(np.array([0, 1, 1, 0, 1], np.int32), torch.int32)
+    Br = _dev(rng.normal(size=(nb, 3)), torch.float64)
+    base = torch.empty((2, n, n), dtype=torch.complex128, device="cuda")
+    ids = _dev(np.arange(2, dtype=np.int32), torch.int32)
+    N.check(L.msp_build_h(n, 2, dsys.data_ptr(), dterms.data_ptr(), ids.data_ptr(), None, None, 0,
+                          base.data_ptr(), eng.stream()))
+    H1 = torch.empty((nb, n, n), dtype=torch.complex128, device="cuda")
+    H2 = torch.empty_like(H1)
+    N.check(L.msp_build_h(n, nb, dsys.data_ptr(), dterms.data_ptr(), sid.data_ptr(), Br.data_ptr(),
+                          None, 1, H1.data_ptr(), eng.stream()))
+    N.check(L.msp_build_h(n, nb, dsys.data_ptr(), None, sid.data_ptr(), Br.data_ptr(),
+                          base.data_ptr(), 1, H2.data_ptr(), eng.stream()))
+    torch.cuda.synchronize()
+    H1, H2, Brh = H1.cpu().numpy(), H2.cpu().numpy(), Br.cpu().numpy()
+    for k, s in enumerate([0, 1, 1, 0, 1]):
+        S = systems[s]
+        ref = physics.hamiltonian(S.Is, S.terms) + physics.zeeman(S.Is, S.gammas, Brh[k])
+        assert np.abs(H1[k] - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert np.abs(H2[k] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_gemm_cn(eng):
     rng = np.random.default_rng(7)
     for n in (1, 5, 32, 47, 192):
