@@ -32,6 +32,13 @@
  *                       cosine sum parallel_fast_time_evolve, cython/parallel.pyx:16-69
  *   msp_integral        Hamiltonian.integrate_decaying, hamiltonian.py:150-162
  *   msp_phase_evolve    evolve without operators (density matrices), hamiltonian.py:108-116
+ *   msp_liouvillian     Lindbladian.from_hamiltonian + add_dissipative_term,
+ *                       lindbladian.py:19-41; experiment.py:271-341
+ *   msp_lindblad_modes  Lindbladian.evolve / integrate_decaying eigendecomposition
+ *                       (numpy.linalg.eig + solve), lindbladian.py:83-97,145-160
+ *   msp_mode_sum        Lindbladian.evolve expectation values, lindbladian.py:99-103
+ *   msp_mode_integral   Lindbladian.integrate_decaying, lindbladian.py:163-171
+ *   msp_mode_states     Lindbladian.evolve density matrices, lindbladian.py:105-109
  *   msp_accumulate      Re(data)*w and store_time_slice powder accumulation,
  *                       experiment.py:498; simconfig.py:347-368
  */
@@ -144,6 +151,45 @@ int msp_integral(int n, int batch, const double *d_E, const double *d_C, int rea
  * returns when no operator is given (muspinsim/hamiltonian.py:108-116). */
 int msp_phase_evolve(int n, int ntimes, const double *d_E, const double *d_rho,
                      const double *d_times, double *d_out, void *stream);
+
+/* Liouvillian of each instance on row-major vec(rho) (N = n^2, n^2 <= 1024):
+ * L = -i (H (x) 1 - 1 (x) H^T)
+ *     + sum_k g_k (A_k (x) conj(A_k) - 1/2 (A_k^H A_k (x) 1 + 1 (x) (A_k^H A_k)^T)).
+ * d_H: batch x n x n; d_A: batch x n_jump x n x n jump operators; d_rate:
+ * batch x n_jump rates g_k; d_L: batch x N x N (complex). */
+int msp_liouvillian(int n, int batch, const double *d_H, int n_jump, const double *d_A,
+                    const double *d_rate, double *d_L, void *stream);
+
+/* Workspace bytes of msp_lindblad_modes (0 when N <= 64: LDS only). */
+size_t msp_lindblad_workspace(int N, int batch);
+
+/* Eigendecomposition L = V diag(lambda) V^{-1} of each instance (d_L is
+ * destroyed) and the mode amplitudes of n_ops observables:
+ *   d_lam  batch x N         eigenvalues lambda_m
+ *   d_amp  batch x n_ops x N amp_km = (u_k^T V)_m (V^{-1} r)_m
+ *   d_c    batch x N         V^{-1} r (optional, may be NULL)
+ *   d_V    batch x N x N     eigenvector columns (optional, may be NULL)
+ * with d_r = vec(rho0) (batch x N) and d_u = vec(O_k^T) (batch x n_ops x N).
+ * d_info[b] = 0 on success, else the index of an unconverged eigenvalue. */
+int msp_lindblad_modes(int N, int batch, double *d_L, const double *d_r, int n_ops,
+                       const double *d_u, double *d_lam, double *d_amp, double *d_c,
+                       double *d_V, int32_t *d_info, void *d_work, size_t work_bytes,
+                       void *stream);
+
+/* P_bk(t) = sum_m amp_bkm exp(2 pi lambda_bm t).  out_complex = 1: d_out is
+ * batch x n_ops x T complex; 0: d_out[(b n_ops + k) T + t] = w_b Re P. */
+int msp_mode_sum(int N, int batch, int n_ops, int ntimes, const double *d_lam,
+                 const double *d_amp, const double *d_times, const double *d_w,
+                 int out_complex, double *d_out, void *stream);
+
+/* I_bk = out_scale sum_m amp_bkm / (1/tau - 2 pi lambda_bm); complex or w_b Re. */
+int msp_mode_integral(int N, int batch, int n_ops, const double *d_lam, const double *d_amp,
+                      double tau, double out_scale, const double *d_w, int out_complex,
+                      double *d_out, void *stream);
+
+/* One instance: d_out[t] = V (c exp(2 pi lambda t)) (N complex per time, ntimes <= 65535). */
+int msp_mode_states(int N, int ntimes, const double *d_lam, const double *d_c, const double *d_V,
+                    const double *d_times, double *d_out, void *stream);
 
 /* results[s][t] += rows[i][t] * scale for i in d_idx[d_ptr[s] .. d_ptr[s+1]),
  * in that order (deterministic powder accumulation). */

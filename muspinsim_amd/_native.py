@@ -57,6 +57,12 @@ SIGNATURES = {
     "msp_timesum": (_I, [_I, _I, _I, _P, _P, _I, _P, _P, _I, _P, _P]),
     "msp_integral": (_I, [_I, _I, _P, _P, _I, _D, _D, _P, _I, _P, _P]),
     "msp_phase_evolve": (_I, [_I, _I, _P, _P, _P, _P, _P]),
+    "msp_liouvillian": (_I, [_I, _I, _P, _I, _P, _P, _P, _P]),
+    "msp_lindblad_workspace": (_SZ, [_I, _I]),
+    "msp_lindblad_modes": (_I, [_I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "msp_mode_sum": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P]),
+    "msp_mode_integral": (_I, [_I, _I, _I, _P, _P, _D, _D, _P, _I, _P, _P]),
+    "msp_mode_states": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
     "msp_profile_enable": (_I, [_I]),
     "msp_profile_report": (_I, [ctypes.c_char_p, _SZ]),

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "msp_jacobi.h"
+#include "msp_nonsym.h"
 #include "msp_tridiag.h"
 
 extern "C" {
@@ -118,5 +119,41 @@ int hc_eigh_dc(int n, const double *H, double *E, double *U, int cw)
     double *Q = stedc_team(n, E, e.data(), al.W, red);
     backtransform_team(n, A.data(), tau.data(), Q, (cdbl *)U, Uc.data(), cw, ws.data());
     return 0;
+}
+
+// Non-Hermitian eigendecomposition: L (n x n complex, copied) -> lam (n
+// complex), V (n x n complex, columns = right eigenvectors), and the mode
+// amplitudes c = V^{-1} r, o = u^T V.  Returns the Schur status (0 = ok).
+int hc_eig(int n, const double *L, double *lam, double *V, const double *r, const double *u,
+           double *c, double *o, int max_its)
+{
+    std::vector<cdbl> A((size_t)n * n), Z((size_t)n * n), v(n), w(n), y(n), t(n);
+    for (auto &x : A) x = cmk(NAN, NAN);
+    for (auto &x : Z) x = cmk(NAN, NAN);
+    memcpy(A.data(), L, sizeof(cdbl) * n * n);
+    double red[1];
+    hessenberg_team(n, A.data(), n, Z.data(), n, v.data(), w.data(), red);
+    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its);
+    if (info) return info;
+    schur_vectors_team(n, A.data(), n, (const cdbl *)lam);
+    modes_coeff_team(n, A.data(), n, Z.data(), n, (const cdbl *)r, y.data());
+    memcpy(c, y.data(), sizeof(cdbl) * n);
+    modes_proj_team(n, A.data(), n, Z.data(), n, (const cdbl *)u, t.data(), (cdbl *)o);
+    modes_vectors_team(n, A.data(), n, Z.data(), n);
+    memcpy(V, Z.data(), sizeof(cdbl) * n * n);
+    return 0;
+}
+
+// Hessenberg + Schur only: T (upper triangular, lower part zeroed) and Z.
+int hc_schur(int n, const double *L, double *T, double *Zout, double *lam, int max_its)
+{
+    std::vector<cdbl> A((size_t)n * n), Z((size_t)n * n), v(n), w(n);
+    memcpy(A.data(), L, sizeof(cdbl) * n * n);
+    double red[1];
+    hessenberg_team(n, A.data(), n, Z.data(), n, v.data(), w.data(), red);
+    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its);
+    memcpy(T, A.data(), sizeof(cdbl) * n * n);
+    memcpy(Zout, Z.data(), sizeof(cdbl) * n * n);
+    return info;
 }
 }

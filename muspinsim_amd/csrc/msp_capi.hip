@@ -31,6 +31,8 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
 constexpr int kMaxDim = 2048;
 constexpr int kJacobiMax = 64;
+constexpr int kMaxLiouville = 1024;      // N = n^2 of the Lindbladian path
+constexpr int kSchurIts = 30;            // QR sweeps per eigenvalue
 
 }  // namespace
 
@@ -222,6 +224,87 @@ int msp_phase_evolve(int n, int ntimes, const double *d_E, const double *d_rho,
         return fail(MSP_ERR_VALUE, "msp_phase_evolve: null pointer");
     return hip_status(launch_phase_evolve(n, ntimes, d_E, d_rho, d_times, d_out, S(stream)),
                       "msp_phase_evolve");
+}
+
+int msp_liouvillian(int n, int batch, const double *d_H, int n_jump, const double *d_A,
+                    const double *d_rate, double *d_L, void *stream)
+{
+    if (n < 1 || n * n > kMaxLiouville)
+        return fail(MSP_ERR_VALUE, "msp_liouvillian: n = %d out of range (n^2 <= %d)", n,
+                    kMaxLiouville);
+    if (batch < 0 || n_jump < 0) return fail(MSP_ERR_VALUE, "msp_liouvillian: negative size");
+    if (batch == 0) return MSP_OK;
+    if (!d_H || !d_L || (n_jump > 0 && (!d_A || !d_rate)))
+        return fail(MSP_ERR_VALUE, "msp_liouvillian: null pointer");
+    return hip_status(launch_liouvillian(n, batch, d_H, n_jump, d_A, d_rate, d_L, S(stream)),
+                      "msp_liouvillian");
+}
+
+size_t msp_lindblad_workspace(int N, int batch)
+{
+    if (N < 1 || batch < 1) return 0;
+    return lindblad_modes_workspace(N, batch);
+}
+
+int msp_lindblad_modes(int N, int batch, double *d_L, const double *d_r, int n_ops,
+                       const double *d_u, double *d_lam, double *d_amp, double *d_c, double *d_V,
+                       int32_t *d_info, void *d_work, size_t work_bytes, void *stream)
+{
+    if (N < 1 || N > kMaxLiouville)
+        return fail(MSP_ERR_VALUE, "msp_lindblad_modes: N = %d out of range", N);
+    if (batch < 0 || n_ops < 0) return fail(MSP_ERR_VALUE, "msp_lindblad_modes: negative size");
+    if (batch == 0) return MSP_OK;
+    if (!d_L || !d_r || !d_lam || !d_info || (n_ops > 0 && (!d_u || !d_amp)))
+        return fail(MSP_ERR_VALUE, "msp_lindblad_modes: null pointer");
+    const size_t need = lindblad_modes_workspace(N, batch);
+    if (need > 0 && (!d_work || work_bytes < need))
+        return fail(MSP_ERR_VALUE, "msp_lindblad_modes: workspace of %zu bytes needed", need);
+    return hip_status(launch_lindblad_modes(N, batch, d_L, d_r, n_ops, d_u, d_lam, d_amp, d_c, d_V,
+                                            d_info, (double *)d_work, kSchurIts, S(stream)),
+                      "msp_lindblad_modes");
+}
+
+int msp_mode_sum(int N, int batch, int n_ops, int ntimes, const double *d_lam, const double *d_amp,
+                 const double *d_times, const double *d_w, int out_complex, double *d_out,
+                 void *stream)
+{
+    if (N < 1 || N > kMaxLiouville) return fail(MSP_ERR_VALUE, "msp_mode_sum: N out of range");
+    if (batch < 0 || n_ops < 0 || ntimes < 0)
+        return fail(MSP_ERR_VALUE, "msp_mode_sum: negative size");
+    if (batch == 0 || n_ops == 0 || ntimes == 0) return MSP_OK;
+    if (!d_lam || !d_amp || !d_times || !d_out)
+        return fail(MSP_ERR_VALUE, "msp_mode_sum: null pointer");
+    return hip_status(launch_mode_sum(N, batch, n_ops, ntimes, d_lam, d_amp, d_times, d_w,
+                                      out_complex, d_out, S(stream)),
+                      "msp_mode_sum");
+}
+
+int msp_mode_integral(int N, int batch, int n_ops, const double *d_lam, const double *d_amp,
+                      double tau, double out_scale, const double *d_w, int out_complex,
+                      double *d_out, void *stream)
+{
+    if (N < 1 || N > kMaxLiouville)
+        return fail(MSP_ERR_VALUE, "msp_mode_integral: N out of range");
+    if (!(tau > 0.0)) return fail(MSP_ERR_VALUE, "msp_mode_integral: tau must be > 0");
+    if (batch < 0 || n_ops < 0) return fail(MSP_ERR_VALUE, "msp_mode_integral: negative size");
+    if (batch == 0 || n_ops == 0) return MSP_OK;
+    if (!d_lam || !d_amp || !d_out) return fail(MSP_ERR_VALUE, "msp_mode_integral: null pointer");
+    return hip_status(launch_mode_integral(N, batch, n_ops, d_lam, d_amp, tau, out_scale, d_w,
+                                           out_complex, d_out, S(stream)),
+                      "msp_mode_integral");
+}
+
+int msp_mode_states(int N, int ntimes, const double *d_lam, const double *d_c, const double *d_V,
+                    const double *d_times, double *d_out, void *stream)
+{
+    if (N < 1 || N > kMaxLiouville) return fail(MSP_ERR_VALUE, "msp_mode_states: N out of range");
+    if (ntimes < 0 || ntimes > 65535)
+        return fail(MSP_ERR_VALUE, "msp_mode_states: ntimes = %d out of range", ntimes);
+    if (ntimes == 0) return MSP_OK;
+    if (!d_lam || !d_c || !d_V || !d_times || !d_out)
+        return fail(MSP_ERR_VALUE, "msp_mode_states: null pointer");
+    return hip_status(launch_mode_states(N, ntimes, d_lam, d_c, d_V, d_times, d_out, S(stream)),
+                      "msp_mode_states");
 }
 
 int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,

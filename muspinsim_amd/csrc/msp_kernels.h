@@ -10,6 +10,16 @@
 struct MspSystem;
 struct MspTerm;
 
+// While profiling is enabled (msp_profile_enable), brackets the launches in
+// its scope with two hipEvents on stream s, reported under `name`.
+struct ProfScope {
+    const char *name;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(const char *n, hipStream_t st);
+    ~ProfScope();
+};
+
 size_t jacobi_lds_bytes(int n);
 size_t dc_workspace_doubles(int n);
 int dc_chunk_width(int n);
@@ -48,3 +58,19 @@ std::string prof_report();
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s);
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
                                      hipStream_t s);
+
+// Lindbladian path (msp_lindblad.hip)
+size_t lindblad_modes_workspace(int N, int batch);
+hipError_t launch_liouvillian(int n, int batch, const double *H, int n_jump, const double *A,
+                              const double *rate, double *L, hipStream_t s);
+hipError_t launch_lindblad_modes(int N, int batch, double *L, const double *r, int n_ops,
+                                 const double *u, double *lam, double *amp, double *c, double *V,
+                                 int32_t *info, double *work, int max_its, hipStream_t s);
+hipError_t launch_mode_sum(int N, int batch, int n_ops, int ntimes, const double *lam,
+                           const double *amp, const double *times, const double *w,
+                           int out_complex, double *out, hipStream_t s);
+hipError_t launch_mode_integral(int N, int batch, int n_ops, const double *lam, const double *amp,
+                                double tau, double out_scale, const double *w, int out_complex,
+                                double *out, hipStream_t s);
+hipError_t launch_mode_states(int N, int ntimes, const double *lam, const double *c,
+                              const double *V, const double *times, double *out, hipStream_t s);

@@ -688,24 +688,22 @@ hipEvent_t prof_event()
     return e;
 }
 
-struct ProfScope {
-    const char *name;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    ProfScope(const char *n, hipStream_t st) : name(n), s(st)
-    {
-        if (g_prof && (a = prof_event())) hipEventRecord(a, s);
-    }
-    ~ProfScope()
-    {
-        if (!a) return;
-        hipEvent_t b = prof_event();
-        if (!b) return;
-        hipEventRecord(b, s);
-        g_recs.push_back({name, a, b});
-    }
-};
 }  // namespace
+
+ProfScope::ProfScope(const char *n, hipStream_t st) : name(n), s(st)
+{
+    if (g_prof && (a = prof_event())) hipEventRecord(a, s);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!a) return;
+    hipEvent_t b = prof_event();
+    if (!b) return;
+    hipEventRecord(b, s);
+    g_recs.push_back({name, a, b});
+}
+
 
 void prof_enable(int on) { g_prof = on != 0; }
 
@@ -745,7 +743,6 @@ std::string prof_report()
 // ---------------------------------------------------------------------------
 // Launchers
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
-#define PROF(name) ProfScope prof_scope_(name, s)
 
 size_t dc_workspace_doubles(int n) { return dc_work_doubles(n); }
 

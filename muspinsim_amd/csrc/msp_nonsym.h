@@ -1,0 +1,327 @@
+// Complex non-Hermitian eigendecomposition as team functions (device: one
+// workgroup per matrix; host: one thread), for the Lindbladian path
+// (muspinsim/lindbladian.py:43-173, which calls numpy.linalg.eig and solve).
+//
+//   L = Z T Z^H                Householder Hessenberg reduction, then complex
+//                              single-shift QR sweeps (Givens bulge chase) to
+//                              upper-triangular Schur form T, Z accumulated
+//   T X = X diag(lambda)       X unit upper triangular, column k by back
+//                              substitution (T - lambda_k) x = 0, x_k = 1
+//   V = Z X                    right eigenvectors (columns)
+//
+// The Lindblad expectation values only need the mode amplitudes
+//   c = V^{-1} r = X^{-1} (Z^H r)          (r = vec(rho0), row-major)
+//   o = u^T V   = (u^T Z) X                (u = vec(O^T))
+// so V^{-1} is never formed: Z is unitary and X triangular.
+//
+// Storage: A (n x n, row stride lda) holds L on entry and T on exit with the
+// strictly lower triangle reused for X: X[i][k] (i < k) is kept at A[k][i].
+// Z (n x n, stride ldz) receives the Schur vectors.
+#pragma once
+
+#include "msp_team.h"
+
+// |re| + |im|, the LAPACK cabs1 magnitude used in deflation tests
+MSP_HD double cabs1(cdbl a) { return fabs(a.re) + fabs(a.im); }
+MSP_HD double cabs(cdbl a) { return hypot(a.re, a.im); }
+
+// ---------------------------------------------------------------------------
+// Hessenberg reduction: A <- Q^H A Q (upper Hessenberg), Z <- Q.
+// v, w: team scratch of n complex each; red: team scratch of MSP_NT doubles.
+MSP_HD void hessenberg_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *v, cdbl *w,
+                            double *red)
+{
+    MSP_PFOR(idx, n * n)
+    {
+        const int i = idx / n, j = idx - (idx / n) * n;
+        Z[i * ldz + j] = cmk(i == j ? 1.0 : 0.0, 0.0);
+    }
+    MSP_SYNC();
+    for (int k = 0; k + 2 < n; ++k) {
+        // reflector H = I - tau v v^H with H^H [alpha; x] = [beta; 0]
+        double part = 0.0;
+        MSP_PFOR(i, n - k - 2) part += cabs2(A[(k + 2 + i) * lda + k]);
+        const double xn2 = team_sum(part, red);
+        const cdbl alpha = A[(k + 1) * lda + k];
+        if (xn2 == 0.0 && alpha.im == 0.0) {
+            MSP_SYNC();
+            continue;                                  // already reduced: tau = 0
+        }
+        const double anorm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+        const double beta = (alpha.re >= 0.0) ? -anorm : anorm;
+        const cdbl tau = cmk((beta - alpha.re) / beta, -alpha.im / beta);
+        const cdbl scal = cdivc(cmk(1.0, 0.0), cmk(alpha.re - beta, alpha.im));
+        MSP_PFOR(i, n - k - 1)
+        {
+            const int r = k + 1 + i;
+            v[r] = (i == 0) ? cmk(1.0, 0.0) : cmul(A[r * lda + k], scal);
+        }
+        MSP_SYNC();
+        // left: A[k+1.., j] -= conj(tau) v (v^H A[k+1.., j]), columns j > k
+        MSP_PFOR(jj, n - k - 1)
+        {
+            const int j = k + 1 + jj;
+            cdbl s = cmk(0.0, 0.0);
+            for (int r = k + 1; r < n; ++r) cfmac(s, v[r], A[r * lda + j]);
+            w[j] = cmul(cconj(tau), s);
+        }
+        MSP_SYNC();
+        MSP_PFOR(idx, (n - k - 1) * (n - k))
+        {
+            const int r = k + 1 + idx / (n - k), j = k + idx % (n - k);
+            if (j == k) A[r * lda + k] = (r == k + 1) ? cmk(beta, 0.0) : cmk(0.0, 0.0);
+            else A[r * lda + j] = csub(A[r * lda + j], cmul(v[r], w[j]));
+        }
+        MSP_SYNC();
+        // right: A[i, k+1..] -= (A[i, k+1..] v) tau v^H for every row; then Z
+        MSP_PFOR(i, n)
+        {
+            cdbl s = cmk(0.0, 0.0);
+            for (int r = k + 1; r < n; ++r) cfma(s, A[i * lda + r], v[r]);
+            w[i] = cmul(s, tau);
+        }
+        MSP_SYNC();
+        MSP_PFOR(idx, n * (n - k - 1))
+        {
+            const int i = idx / (n - k - 1), r = k + 1 + idx % (n - k - 1);
+            A[i * lda + r] = csub(A[i * lda + r], cmul(w[i], cconj(v[r])));
+        }
+        MSP_SYNC();
+        MSP_PFOR(i, n)
+        {
+            cdbl s = cmk(0.0, 0.0);
+            for (int r = k + 1; r < n; ++r) cfma(s, Z[i * ldz + r], v[r]);
+            w[i] = cmul(s, tau);
+        }
+        MSP_SYNC();
+        MSP_PFOR(idx, n * (n - k - 1))
+        {
+            const int i = idx / (n - k - 1), r = k + 1 + idx % (n - k - 1);
+            Z[i * ldz + r] = csub(Z[i * ldz + r], cmul(w[i], cconj(v[r])));
+        }
+        MSP_SYNC();
+    }
+}
+
+// Givens rotation G = [c s; -conj(s) c] with G [f; g] = [r; 0], c real.
+MSP_HD void givens(cdbl f, cdbl g, double &c, cdbl &s, cdbl &r)
+{
+    if (g.re == 0.0 && g.im == 0.0) {
+        c = 1.0;
+        s = cmk(0.0, 0.0);
+        r = f;
+        return;
+    }
+    if (f.re == 0.0 && f.im == 0.0) {
+        const double ga = cabs(g);
+        c = 0.0;
+        s = cmk(g.re / ga, -g.im / ga);
+        r = cmk(ga, 0.0);
+        return;
+    }
+    const double fa = cabs(f), ga = cabs(g);
+    const double nrm = hypot(fa, ga);
+    const cdbl fs = cmk(f.re / fa, f.im / fa);       // f / |f|
+    c = fa / nrm;
+    s = cscale(cmul(fs, cconj(g)), 1.0 / nrm);
+    r = cscale(fs, nrm);
+}
+
+// ---------------------------------------------------------------------------
+// Upper Hessenberg A -> upper triangular T = Z^H A Z by single-shift QR with
+// Wilkinson shifts (exceptional shifts after 10 and 20 stalled sweeps).
+// lam receives the diagonal.  Returns 0, or the 1-based index of an
+// unconverged eigenvalue after max_its sweeps per eigenvalue.
+MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int max_its)
+{
+    const double ulp = 1.1102230246251565e-16;
+    const double smlnum = 2.2250738585072014e-308 * ((double)n / ulp);
+    int ihi = n - 1, its = 0, total = 0;
+    while (ihi >= 0) {
+        // split point: largest l <= ihi with A[l][l-1] negligible
+        int l = ihi;
+        for (; l > 0; --l) {
+            const double sub = cabs1(A[l * lda + l - 1]);
+            if (sub <= smlnum) break;
+            double tst = cabs1(A[(l - 1) * lda + l - 1]) + cabs1(A[l * lda + l]);
+            if (tst == 0.0) {
+                if (l >= 2) tst += fabs(A[(l - 1) * lda + l - 2].re);
+                if (l + 1 <= ihi) tst += fabs(A[(l + 1) * lda + l].re);
+            }
+            if (sub <= ulp * tst) break;
+        }
+        MSP_SYNC();
+        if (l > 0) {
+            if (MSP_TID == 0) A[l * lda + l - 1] = cmk(0.0, 0.0);
+        }
+        if (l == ihi) {
+            if (MSP_TID == 0) lam[ihi] = A[ihi * lda + ihi];
+            MSP_SYNC();
+            --ihi;
+            its = 0;
+            continue;
+        }
+        if (++total > max_its * n) return ihi + 1;
+        // shift
+        cdbl sft;
+        if (its == 10) {
+            const cdbl a = A[l * lda + l];
+            sft = cmk(a.re + 0.75 * fabs(A[(l + 1) * lda + l].re), a.im);
+        } else if (its == 20) {
+            const cdbl a = A[ihi * lda + ihi];
+            sft = cmk(a.re + 0.75 * fabs(A[ihi * lda + ihi - 1].re), a.im);
+        } else {
+            // eigenvalue of the trailing 2 x 2 block closest to its last diagonal
+            const cdbl a = A[(ihi - 1) * lda + ihi - 1], b = A[(ihi - 1) * lda + ihi];
+            const cdbl c = A[ihi * lda + ihi - 1], d = A[ihi * lda + ihi];
+            const cdbl hm = cscale(csub(a, d), 0.5);
+            const cdbl disc = cadd(cmul(hm, hm), cmul(b, c));
+            // principal square root
+            const double m = cabs(disc);
+            double sr = sqrt(0.5 * (m + fabs(disc.re)));
+            double si = (sr > 0.0) ? 0.5 * disc.im / sr : 0.0;
+            if (disc.re < 0.0) {
+                const double t = sr;
+                sr = fabs(si);
+                si = (disc.im >= 0.0) ? t : -t;
+            }
+            cdbl root = cmk(sr, si);
+            // lambda = d + hm -/+ root; pick the one nearer d: d + (hm - root) or d + (hm + root)
+            const cdbl e1 = csub(hm, root), e2 = cadd(hm, root);
+            sft = cadd(d, (cabs1(e1) <= cabs1(e2)) ? e1 : e2);
+        }
+        ++its;
+        // single-shift sweep over rows/cols l .. ihi, applied to the full
+        // matrix (columns up to n-1, rows from 0) and to Z
+        for (int k = l; k < ihi; ++k) {
+            cdbl f, g;
+            if (k == l) {
+                f = csub(A[l * lda + l], sft);
+                g = A[(l + 1) * lda + l];
+            } else {
+                f = A[k * lda + k - 1];
+                g = A[(k + 1) * lda + k - 1];
+            }
+            double c;
+            cdbl s, r;
+            givens(f, g, c, s, r);
+            MSP_SYNC();
+            // rows k, k+1: [a; b] <- [c a + s b; -conj(s) a + c b]
+            const int j0 = (k == l) ? k : k - 1;
+            MSP_PFOR(jj, n - j0)
+            {
+                const int j = j0 + jj;
+                const cdbl a = A[k * lda + j], b = A[(k + 1) * lda + j];
+                if (k > l && j == k - 1) {
+                    A[k * lda + j] = r;
+                    A[(k + 1) * lda + j] = cmk(0.0, 0.0);
+                } else {
+                    A[k * lda + j] = cadd(cscale(a, c), cmul(s, b));
+                    A[(k + 1) * lda + j] = csub(cscale(b, c), cmul(cconj(s), a));
+                }
+            }
+            MSP_SYNC();
+            // columns k, k+1 of A (rows 0 .. min(k+2, ihi)) and of Z (all rows):
+            // [p q] <- [c p + conj(s) q, -s p + c q]
+            const int iend = (k + 2 < ihi) ? k + 2 : ihi;
+            MSP_PFOR(i2, (iend + 1) + n)
+            {
+                cdbl *M;
+                int ld, i;
+                if (i2 <= iend) { M = A; ld = lda; i = i2; }
+                else { M = Z; ld = ldz; i = i2 - iend - 1; }
+                const cdbl p = M[i * ld + k], q = M[i * ld + k + 1];
+                M[i * ld + k] = cadd(cscale(p, c), cmul(cconj(s), q));
+                M[i * ld + k + 1] = csub(cscale(q, c), cmul(s, p));
+            }
+            MSP_SYNC();
+        }
+    }
+    MSP_SYNC();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Eigenvectors of the Schur form: for each k, X[i][k] (i < k) with
+// (T - lambda_k) x = 0, x_k = 1, stored at A[k][i].  Near-zero pivots are
+// replaced by smin = max(ulp |lambda_k|, smlnum) as LAPACK's ztrevc does.
+MSP_HD void schur_vectors_team(int n, cdbl *A, int lda, const cdbl *lam)
+{
+    const double ulp = 1.1102230246251565e-16;
+    const double smlnum = 2.2250738585072014e-308 * ((double)n / ulp);
+    MSP_PFOR(k, n)
+    {
+        const cdbl lk = lam[k];
+        const double smin = fmax(ulp * cabs1(lk), smlnum);
+        // x_i for i = k-1 .. 0; x stored in the lower triangle row k
+        for (int i = k - 1; i >= 0; --i) {
+            cdbl s = A[i * lda + k];                  // T_ik * x_k (x_k = 1)
+            for (int j = i + 1; j < k; ++j) cfma(s, A[i * lda + j], A[k * lda + j]);
+            cdbl den = csub(lam[i], lk);
+            if (cabs1(den) < smin) den = cmk(smin, 0.0);
+            A[k * lda + i] = cscale(cdivc(s, den), -1.0);
+        }
+    }
+    MSP_SYNC();
+}
+
+// X[i][k] accessor (unit diagonal, zero below)
+MSP_HD cdbl xel(const cdbl *A, int lda, int i, int k)
+{
+    if (i == k) return cmk(1.0, 0.0);
+    if (i > k) return cmk(0.0, 0.0);
+    return A[k * lda + i];
+}
+
+// c = X^{-1} (Z^H r).  y: team scratch (n); out: c (n).
+MSP_HD void modes_coeff_team(int n, const cdbl *A, int lda, const cdbl *Z, int ldz, const cdbl *r,
+                             cdbl *y)
+{
+    MSP_PFOR(m, n)
+    {
+        cdbl s = cmk(0.0, 0.0);
+        for (int i = 0; i < n; ++i) cfmac(s, Z[i * ldz + m], r[i]);
+        y[m] = s;
+    }
+    MSP_SYNC();
+    // back substitution with the unit upper triangular X, column oriented
+    for (int j = n - 1; j > 0; --j) {
+        const cdbl cj = y[j];
+        MSP_PFOR(i, j) y[i] = csub(y[i], cmul(A[j * lda + i], cj));
+        MSP_SYNC();
+    }
+}
+
+// o = (u^T Z) X.  t: team scratch (n); out o (n).
+MSP_HD void modes_proj_team(int n, const cdbl *A, int lda, const cdbl *Z, int ldz, const cdbl *u,
+                            cdbl *t, cdbl *o)
+{
+    MSP_PFOR(m, n)
+    {
+        cdbl s = cmk(0.0, 0.0);
+        for (int i = 0; i < n; ++i) cfma(s, u[i], Z[i * ldz + m]);
+        t[m] = s;
+    }
+    MSP_SYNC();
+    MSP_PFOR(k, n)
+    {
+        cdbl s = t[k];
+        for (int j = 0; j < k; ++j) cfma(s, t[j], A[k * lda + j]);
+        o[k] = s;
+    }
+    MSP_SYNC();
+}
+
+// V = Z X in place in Z (row by row, columns in descending order).
+MSP_HD void modes_vectors_team(int n, const cdbl *A, int lda, cdbl *Z, int ldz)
+{
+    MSP_PFOR(i, n)
+    {
+        for (int k = n - 1; k >= 0; --k) {
+            cdbl s = Z[i * ldz + k];
+            for (int j = 0; j < k; ++j) cfma(s, Z[i * ldz + j], A[k * lda + j]);
+            Z[i * ldz + k] = s;
+        }
+    }
+    MSP_SYNC();
+}

@@ -227,17 +227,21 @@ class SpinEngine:
         plan = self.prepare(systems, batch)
         res = self.execute(plan)
         out = res.cpu().numpy()
+        self.check_converged(plan)
         if return_rows:
             return out, plan.rows.cpu().numpy()
         return out
 
+    @staticmethod
+    def check_converged(plan):
+        """LinAlgError if any eigensolver of the last execute did not converge
+        (synchronises with the device)."""
+        if plan.checks and int(torch.stack(plan.checks).max().item()) != 0:
+            raise np.linalg.LinAlgError("Eigenvalues did not converge")
+
     def prepare(self, systems, batch):
         """Route instances (experiment.py:413-431), upload every per-instance
         array and allocate the chunk buffers; nothing is computed."""
-        for s in systems:
-            if s.dissip:
-                raise NotImplementedError(
-                    "dissipative (Lindblad) instances are not handled by SpinEngine.run")
         nb = len(batch)
         width = 1 if batch.y == "integral" else len(batch.times)
         plan = _Plan()
@@ -247,6 +251,7 @@ class SpinEngine:
         plan.results = torch.zeros((batch.n_slots, width), dtype=torch.float64, device=self.device)
         plan.rows = torch.zeros((max(nb, 1), width), dtype=torch.float64, device=self.device)
         plan.groups = []
+        plan.checks = []
         if nb == 0:
             return plan
         Br_h = rotate_host(batch.q, batch.B) + batch.Bint
@@ -256,7 +261,9 @@ class SpinEngine:
             check = (cnst.e * (cnst.hbar ** 2) * Bn) / (2 * cnst.m_p * cnst.k * Tarr)
         iso = check == 0                       # every non-muon spin at infinite temperature
         mu_first = np.array([systems[int(k)].muon_index == 0 for k in batch.sys_id])
-        fast = iso & mu_first & (batch.y == "asymmetry")
+        lind = np.array([bool(systems[int(k)].dissip) for k in batch.sys_id])
+        fast = iso & mu_first & (batch.y == "asymmetry") & ~lind
+        iso = iso & ~lind
         ns = np.array([systems[int(k)].n for k in batch.sys_id])
         sysarr, termarr = pack_systems(systems)
         plan.d_sys = self._bytes_view(sysarr)
@@ -264,23 +271,32 @@ class SpinEngine:
         plan.times = self._dev(batch.times if batch.y == "asymmetry" else np.zeros(1))
         for n in np.unique(ns):
             n = int(n)
-            for route in ("fast", "iso", "general"):
+            for route in ("fast", "iso", "general", "lindblad"):
                 if route == "fast":
                     sel = (ns == n) & fast
                 elif route == "iso":
                     sel = (ns == n) & iso & ~fast
+                elif route == "general":
+                    sel = (ns == n) & ~iso & ~lind
                 else:
-                    sel = (ns == n) & ~iso
+                    sel = (ns == n) & lind
                 idx = np.nonzero(sel)[0]
                 if idx.size == 0:
                     continue
-                if route == "general":
+                if route in ("general", "lindblad"):
                     dims0 = systems[int(batch.sys_id[idx[0]])].dims
                     for k in idx:
                         if systems[int(batch.sys_id[k])].dims != dims0:
                             raise NotImplementedError(
                                 "finite-temperature instances of one dimension must share spin dims")
-                cs = self.chunk_size(n, route == "general")
+                if route == "lindblad":
+                    if n * n > 1024:
+                        raise NotImplementedError(
+                            "dissipative systems are limited to n^2 <= 1024 (Liouville space)")
+                    cs = int(max(1, min(self.max_chunk, self.chunk_bytes //
+                                        (3 * n ** 4 * 16 + 8 * n * n * 16))))
+                else:
+                    cs = self.chunk_size(n, route == "general")
                 g = _Group()
                 g.n, g.route = n, route
                 g.cs = min(cs, idx.size)
@@ -296,6 +312,13 @@ class SpinEngine:
                 if route == "general":
                     g.F = self._dev(rho0_factors_host(systems, batch, idx, Br_h), torch.complex128)
                     g.dims = systems[int(batch.sys_id[idx[0]])].dims
+                if route == "lindblad":
+                    r, u, A, rates = lindblad_inputs_host(systems, batch, idx, Br_h)
+                    g.r = self._dev(r, torch.complex128)
+                    g.u = self._dev(u, torch.complex128)
+                    g.A = self._dev(A, torch.complex128)
+                    g.rates = self._dev(rates)
+                    g.n_jump = A.shape[1]
                 if route == "fast":
                     s0 = systems[int(batch.sys_id[idx[0]])]
                     g.d_other = n // s0.dims[s0.muon_index]
@@ -319,6 +342,18 @@ class SpinEngine:
 
     def _alloc(self, g):
         n, cs, dev = g.n, g.cs, self.device
+        if g.route == "lindblad":
+            Nl = n * n
+            g.Br = torch.empty((cs, 3), dtype=torch.float64, device=dev)
+            g.pr = torch.empty((cs, 3), dtype=torch.float64, device=dev)
+            g.H = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
+            g.L = torch.empty((cs, Nl, Nl), dtype=torch.complex128, device=dev)
+            g.lam = torch.empty((cs, Nl), dtype=torch.complex128, device=dev)
+            g.amp = torch.empty((cs, 1, Nl), dtype=torch.complex128, device=dev)
+            g.info = torch.empty((cs,), dtype=torch.int32, device=dev)
+            g.wsb = self.lib.msp_lindblad_workspace(Nl, cs)
+            g.work = torch.empty((max(g.wsb, 8) // 8,), dtype=torch.float64, device=dev)
+            return
         g.Br = torch.empty((cs, 3), dtype=torch.float64, device=dev)
         g.pr = torch.empty((cs, 3), dtype=torch.float64, device=dev)
         g.H = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
@@ -341,6 +376,7 @@ class SpinEngine:
         st = self.stream()
         batch = plan.batch
         plan.results.zero_()
+        plan.checks = []
         for g in plan.groups:
             n = g.n
             if g.base is not None:
@@ -363,8 +399,12 @@ class SpinEngine:
                 else:
                     N.check(L.msp_build_h(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
                                           p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
+                if g.route == "lindblad":
+                    self._execute_lindblad(plan, g, sl, nb, st)
+                    continue
                 N.check(L.msp_eigh(n, nb, g.H.data_ptr(), g.E.data_ptr(), g.U.data_ptr(),
                                    g.info.data_ptr(), self.method, g.work.data_ptr(), g.wsb, st))
+                plan.checks.append(g.info[:nb].amax())
                 Y = g.H   # destroyed by the eigensolver; reused as scratch
                 pauli = 1 if g.route == "fast" else 0
                 N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
@@ -400,6 +440,92 @@ class SpinEngine:
         return plan.results
 
 
+    def _execute_lindblad(self, plan, g, sl, nb, st):
+        """Liouvillian -> modes -> mode sum / integral for one chunk of a
+        dissipative group (lindbladian.py:43-173 via experiment.py:489-496)."""
+        L = self.lib
+        n = g.n
+        Nl = n * n
+
+        def p(t):
+            return t[sl].data_ptr()
+
+        N.check(L.msp_liouvillian(n, nb, g.H.data_ptr(), g.n_jump, p(g.A), p(g.rates),
+                                  g.L.data_ptr(), st))
+        N.check(L.msp_lindblad_modes(Nl, nb, g.L.data_ptr(), p(g.r), 1, p(g.u), g.lam.data_ptr(),
+                                     g.amp.data_ptr(), None, None, g.info.data_ptr(),
+                                     g.work.data_ptr(), g.wsb, st))
+        plan.checks.append(g.info[:nb].amax())
+        out = torch.empty((nb, plan.width), dtype=torch.float64, device=self.device)
+        if plan.batch.y == "integral":
+            N.check(L.msp_mode_integral(Nl, nb, 1, g.lam.data_ptr(), g.amp.data_ptr(), MU_TAU,
+                                        1.0 / MU_TAU, p(g.w), 0, out.data_ptr(), st))
+        else:
+            N.check(L.msp_mode_sum(Nl, nb, 1, plan.width, g.lam.data_ptr(), g.amp.data_ptr(),
+                                   plan.times.data_ptr(), p(g.w), 0, out.data_ptr(), st))
+        plan.rows.index_copy_(0, g.d_idx[sl], out)
+
+    # ------------------------------------------------------------- Lindblad
+    def liouvillian(self, H, A=None, rates=None):
+        """L (B, n^2, n^2) of Hamiltonians H (B, n, n) and jump operators
+        A (B, k, n, n) with rates (B, k)."""
+        B, n, _ = H.shape
+        k = 0 if A is None else A.shape[1]
+        Lm = torch.empty((B, n * n, n * n), dtype=torch.complex128, device=self.device)
+        N.check(self.lib.msp_liouvillian(n, B, H.data_ptr(), k, None if A is None else A.data_ptr(),
+                                         None if rates is None else rates.data_ptr(),
+                                         Lm.data_ptr(), self.stream()))
+        return Lm
+
+    def lindblad_modes(self, Lm, r, u=None, want_c=False, want_V=False):
+        """lambda (B, N), amp (B, k, N), c (B, N) or None, V (B, N, N) or None;
+        Lm is destroyed.  Raises LinAlgError if a QR iteration did not converge."""
+        B, Nl, _ = Lm.shape
+        k = 0 if u is None else u.shape[1]
+        dev = self.device
+        lam = torch.empty((B, Nl), dtype=torch.complex128, device=dev)
+        amp = torch.empty((B, max(k, 1), Nl), dtype=torch.complex128, device=dev)
+        c = torch.empty((B, Nl), dtype=torch.complex128, device=dev) if want_c else None
+        V = torch.empty((B, Nl, Nl), dtype=torch.complex128, device=dev) if want_V else None
+        info = torch.empty((B,), dtype=torch.int32, device=dev)
+        wsb = self.lib.msp_lindblad_workspace(Nl, B)
+        work = torch.empty((max(wsb, 8) // 8,), dtype=torch.float64, device=dev)
+        N.check(self.lib.msp_lindblad_modes(
+            Nl, B, Lm.data_ptr(), r.data_ptr(), k, None if u is None else u.data_ptr(),
+            lam.data_ptr(), amp.data_ptr(), None if c is None else c.data_ptr(),
+            None if V is None else V.data_ptr(), info.data_ptr(), work.data_ptr(), wsb,
+            self.stream()))
+        if int(info.max().item()) != 0:
+            raise np.linalg.LinAlgError("Eigenvalues did not converge")
+        return lam, amp, c, V
+
+    def mode_sum(self, lam, amp, times):
+        """(B, k, T) complex: sum_m amp_km exp(2 pi lambda_m t)."""
+        B, k, Nl = amp.shape
+        T = times.shape[0]
+        out = torch.empty((B, k, T), dtype=torch.complex128, device=self.device)
+        N.check(self.lib.msp_mode_sum(Nl, B, k, T, lam.data_ptr(), amp.data_ptr(),
+                                      times.data_ptr(), None, 1, out.data_ptr(), self.stream()))
+        return out
+
+    def mode_integral(self, lam, amp, tau):
+        """(B, k) complex: sum_m amp_km / (1/tau - 2 pi lambda_m)."""
+        B, k, Nl = amp.shape
+        out = torch.empty((B, k), dtype=torch.complex128, device=self.device)
+        N.check(self.lib.msp_mode_integral(Nl, B, k, lam.data_ptr(), amp.data_ptr(), float(tau),
+                                           1.0, None, 1, out.data_ptr(), self.stream()))
+        return out
+
+    def mode_states(self, lam, c, V, times):
+        """(T, N) complex: V (c exp(2 pi lambda t)) for one instance."""
+        Nl = lam.shape[-1]
+        T = times.shape[0]
+        out = torch.empty((T, Nl), dtype=torch.complex128, device=self.device)
+        N.check(self.lib.msp_mode_states(Nl, T, lam.data_ptr(), c.data_ptr(), V.data_ptr(),
+                                         times.data_ptr(), out.data_ptr(), self.stream()))
+        return out
+
+
 class _Plan:
     pass
 
@@ -412,6 +538,80 @@ def ctypes_int_array(vals):
     import ctypes
     arr = (ctypes.c_int32 * len(vals))(*[int(v) for v in vals])
     return ctypes.cast(arr, ctypes.c_void_p)
+
+
+def _full_spin_ops(Is, i):
+    """(3, n, n) dense S_x, S_y, S_z of spin i on the whole system."""
+    from .spinop import spin_matrices
+    dims = [int(round(2 * I + 1)) for I in Is]
+    out = []
+    for S in spin_matrices(Is[i]):
+        M = np.ones((1, 1), dtype=complex)
+        for k, d in enumerate(dims):
+            M = np.kron(M, S if k == i else np.eye(d))
+        out.append(M)
+    return np.array(out)
+
+
+def _perp_axes(Bv):
+    """(x, y) perpendicular to each field vector; (x^, y^) for B = 0
+    (experiment.py:296-301, utils.py:71-93)."""
+    nb = len(Bv)
+    x = np.tile([1.0, 0.0, 0.0], (nb, 1))
+    y = np.tile([0.0, 1.0, 0.0], (nb, 1))
+    Bn = np.linalg.norm(Bv, axis=1)
+    for k in np.nonzero(Bn != 0)[0]:
+        z = Bv[k] / Bn[k]
+        if z[0] == 0 and z[1] == 0:
+            x[k] = [1.0, 0.0, 0.0]
+            y[k] = [0.0, z[2], 0.0]
+        else:
+            xx = np.array([z[1], -z[0], 0.0]) / np.hypot(z[0], z[1])
+            x[k] = xx
+            y[k] = np.cross(z, xx)
+    return x, y
+
+
+def lindblad_inputs_host(systems, batch, part, Br_h):
+    """Per-instance inputs of the dissipative route: vec(rho0) (B, N),
+    vec(O^T) of O = p'.S_mu (B, 1, N), jump operators S_i.x +- i S_i.y
+    (B, 2 k, n, n) and their rates (a/pi) Z/(1+Z), (a/pi)/(1+Z),
+    Z = exp(-h gamma_i |B| 1e6 / (k T)) (experiment.py:169-236,271-325)."""
+    s = systems[int(batch.sys_id[part[0]])]
+    n = s.n
+    nb = len(part)
+    F = rho0_factors_host(systems, batch, part, Br_h)
+    rho = np.ones((nb, 1, 1), dtype=complex)
+    off = 0
+    for I in s.Is:
+        d = int(round(2 * I + 1))
+        f = F[:, off:off + d * d].reshape(nb, d, d)
+        off += d * d
+        m = rho.shape[1]
+        rho = np.einsum("bij,bkl->bikjl", rho, f).reshape(nb, m * d, m * d)
+    pr = rotate_host(batch.q[part], batch.p[part])
+    Smu = _full_spin_ops(s.Is, s.muon_index)
+    O = np.einsum("ba,aij->bij", pr, Smu)
+    u = np.swapaxes(O, 1, 2).reshape(nb, 1, n * n)
+    Bv = Br_h[part]
+    Bn = np.linalg.norm(Bv, axis=1)
+    x, y = _perp_axes(Bv)
+    T = np.asarray(batch.T[part], dtype=float)
+    A, rates = [], []
+    for i, a in s.dissip.items():
+        Si = _full_spin_ops(s.Is, i)
+        ox = np.einsum("ba,aij->bij", x, Si)
+        oy = np.einsum("ba,aij->bij", y, Si)
+        with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+            Zu = np.where(T > 0, np.exp(-cnst.h * s.gammas[i] * Bn * 1e6 /
+                                        (cnst.k * np.where(T > 0, T, 1.0))), 0.0)
+        A.append(ox + 1.0j * oy)
+        rates.append(a / np.pi * Zu / (1 + Zu))
+        A.append(ox - 1.0j * oy)
+        rates.append(a / np.pi / (1 + Zu))
+    A = np.stack(A, axis=1)
+    rates = np.stack(rates, axis=1)
+    return rho.reshape(nb, n * n), u, A, rates
 
 
 def rho0_factors_host(systems, batch, part, Br_h):

@@ -11,6 +11,8 @@ orientation weights normalised as simconfig.py:152-159 does.
   c3_tf_stress           mu + 8 19F/1H dipolar (n=512); 0.01 T, p = x, 256 orientations,
                          4000 t in 0-20 us
   c4_uneven_batch        4096 systems, n = 4 .. 1024, 101 t in 0-10 us
+  c4_dissipative         1024 systems mu, e, 1H (n=8, Liouville N=64) with electron
+                         dissipation a ~ U(1, 100) MHz, 2000 t in 0-10 us
   c5_fit                 mu, e, 4 x 1H (n=64); 256 orientations, 2000 t in 0-10 us
 """
 
@@ -191,6 +193,29 @@ def c4_uneven_batch(seed=4, ninst=4096, ntimes=101, max_dim=1024):
     # one averaged output: every instance is its own orientation sample
     batch = _batch(q, np.zeros((ninst, 3)), np.tile([0, 0, 1.0], (ninst, 1)), np.ones(ninst),
                    np.zeros(ninst, dtype=np.int64), np.arange(ninst), times, "asymmetry", 1, ninst)
+    return systems, batch
+
+
+def c4_dissipative(seed=44, ninst=1024, ntimes=2000, B=(0.0, 0.0, 0.0), T=np.inf):
+    """Dissipative subset: per-instance random hyperfine / dipolar couplings
+    and electron dissipation rate, orientation averaged into one output."""
+    rng = np.random.default_rng(seed)
+    Is = [0.5, 0.5, 0.5]
+    g = [MU_GAMMA, ELEC_GAMMA, G_H]
+    systems = []
+    for _ in range(ninst):
+        terms = [("bilinear", 0, 1, rng.uniform(50, 500) * np.eye(3) + _sym_aniso(rng, 10.0)),
+                 ("bilinear", 2, 1, rng.uniform(1, 20) * np.eye(3) + _sym_aniso(rng, 2.0))]
+        r = _random_vector(rng, 1.0, 3.0)
+        terms.append(("bilinear", 0, 2, dipolar_tensor(g[0], g[2], r)))
+        systems.append(SystemDesc(Is, g, terms, 0, dissip={1: float(rng.uniform(1.0, 100.0))}))
+    ang = spiral_orientations(ninst)
+    q = polar_quats_conj(ang)
+    times = np.linspace(0.0, 10.0, ntimes)
+    batch = _batch(q, np.tile(np.asarray(B, dtype=float), (ninst, 1)),
+                   np.tile([0, 0, 1.0], (ninst, 1)), np.ones(ninst),
+                   np.zeros(ninst, dtype=np.int64), np.arange(ninst), times, "asymmetry", 1, ninst,
+                   T=T)
     return systems, batch
 
 
