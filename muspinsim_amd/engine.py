@@ -597,13 +597,20 @@ def lindblad_inputs_host(systems, batch, part, Br_h):
     Bn = np.linalg.norm(Bv, axis=1)
     x, y = _perp_axes(Bv)
     T = np.asarray(batch.T[part], dtype=float)
+    members = [systems[int(j)] for j in batch.sys_id[part]]
+    keys = sorted(s.dissip)
+    if any(sorted(m.dissip) != keys for m in members):
+        raise NotImplementedError("dissipative instances of one dimension must dissipate on "
+                                  "the same spins")
+    gam = np.array([m.gammas for m in members], dtype=float)
     A, rates = [], []
-    for i, a in s.dissip.items():
+    for i in keys:
+        a = np.array([m.dissip[i] for m in members], dtype=float)
         Si = _full_spin_ops(s.Is, i)
         ox = np.einsum("ba,aij->bij", x, Si)
         oy = np.einsum("ba,aij->bij", y, Si)
         with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
-            Zu = np.where(T > 0, np.exp(-cnst.h * s.gammas[i] * Bn * 1e6 /
+            Zu = np.where(T > 0, np.exp(-cnst.h * gam[:, i] * Bn * 1e6 /
                                         (cnst.k * np.where(T > 0, T, 1.0))), 0.0)
         A.append(ox + 1.0j * oy)
         rates.append(a / np.pi * Zu / (1 + Zu))
@@ -618,6 +625,7 @@ def rho0_factors_host(systems, batch, part, Br_h):
     """Per-spin initial states (experiment.py:169-236) as kron factors, packed
     per instance: muon pure along p', others thermal under their Zeeman term."""
     s = systems[int(batch.sys_id[part[0]])]
+    gam = np.array([systems[int(j)].gammas for j in batch.sys_id[part]], dtype=float)
     pr = rotate_host(batch.q[part], batch.p[part])
     out = []
     for k, I in enumerate(s.Is):
@@ -634,7 +642,7 @@ def rho0_factors_host(systems, batch, part, Br_h):
             Bk = Br_h[part]
             Hz = Bk[:, 0, None, None] * Sx + Bk[:, 1, None, None] * Sy + Bk[:, 2, None, None] * Sz
             ev, vec = np.linalg.eigh(Hz)
-            E = ev * 1e6 * s.gammas[k]
+            E = ev * 1e6 * gam[:, k, None]
             T = np.asarray(batch.T[part], dtype=float)[:, None]
             with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
                 Z = np.where(T > 0, np.exp(-cnst.h * E / (cnst.k * np.where(T > 0, T, 1.0))),

@@ -1,0 +1,272 @@
+"""ExperimentRunner of the drop-in surface (muspinsim/experiment.py:21-498).
+
+`run()` carries every configuration of a MuSpinConfig through the batched
+GPU engine in one pass (H build, eigensolver or Lindbladian modes, basis
+change, time sum or integral, powder accumulation); `load_config` /
+`run_single` and the cached properties (B, p, T, rho0, Hz, Htot,
+dissipation_operators, p_operator) evaluate one configuration through the
+Hamiltonian / Lindbladian classes, as the reference does per snapshot.
+"""
+
+import logging
+
+import numpy as np
+import scipy.constants as cnst
+
+from .config import MuSpinConfig
+from .constants import MU_TAU
+from .engine import InstanceBatch, SpinEngine
+from .hamiltonian import Hamiltonian
+from .lindbladian import Lindbladian
+from .orientations import rotate
+from .spinop import DensityOperator, SpinOperator
+
+
+def _perp(z):
+    """Two axes orthogonal to z (utils.py:71-93)."""
+    zn = np.linalg.norm(z)
+    if zn == 0:
+        raise ValueError("Can not find X and Y for null Z vector")
+    z = np.asarray(z, dtype=float) / zn
+    if z[0] == 0 and z[1] == 0:
+        return np.array([1.0, 0, 0]), np.array([0, z[2], 0])
+    x = np.array([z[1], -z[0], 0.0]) / np.hypot(z[0], z[1])
+    return x, np.cross(z, x)
+
+
+class ExperimentRunner:
+    def __init__(self, config, variables=None):
+        """config: a MuSpinConfig; variables: values of fitting variables
+        (when given, run() leaves the results function to the caller, as in
+        experiment.py:60-70)."""
+        if not isinstance(config, MuSpinConfig):
+            raise TypeError("ExperimentRunner needs a MuSpinConfig")
+        self._config = config
+        self._variables = {} if variables is None else dict(variables)
+        self._system = config.system
+        self._B = np.zeros(3)
+        self._p = np.array([1.0, 0, 0])
+        self._T = np.inf
+        self._Hsys = None
+        self._rho0 = None
+        self._Hz = None
+        self._dops = None
+        self._T_inf_speedup = False
+        self._engine = None
+
+    # ---------------------------------------------------------- parameters
+    @property
+    def config(self):
+        return self._config
+
+    @property
+    def system(self):
+        return self._system
+
+    @property
+    def B(self):
+        return self._B
+
+    @B.setter
+    def B(self, x):
+        x = np.array(x, dtype=float)
+        if (x != self._B).any():
+            self._B = x
+            self._rho0 = None
+            self._Hz = None
+            self._dops = None
+
+    @property
+    def p(self):
+        return self._p
+
+    @p.setter
+    def p(self, x):
+        x = np.array(x, dtype=float)
+        if (x != self._p).any():
+            self._p = x
+            self._rho0 = None
+
+    @property
+    def T(self):
+        return self._T
+
+    @T.setter
+    def T(self, x):
+        if x != self._T:
+            self._T = x
+            self._rho0 = None
+            self._dops = None
+
+    # ---------------------------------------------------------- operators
+    @property
+    def rho0(self):
+        """Muon pure along p, every other spin thermal under its own Zeeman
+        term at T (experiment.py:169-236)."""
+        if self._rho0 is None:
+            T, B = self._T, self._B
+            if np.isclose(np.linalg.norm(B), 0.0) and T < np.inf:
+                logging.warning("initial density matrix at zero field and finite temperature "
+                                "uses an approximation that can fail")
+            rho = None
+            for i in range(len(self._system)):
+                I = self._system.I(i)
+                if i == self._system.muon_index:
+                    r = DensityOperator.from_vectors(I, self._p, 0)
+                else:
+                    Hz = sum(B[j] * SpinOperator.from_axes(I, a, use_sparse=False).matrix
+                             for j, a in enumerate("xyz"))
+                    E, V = np.linalg.eigh(Hz)
+                    E = E * 1e6 * self._system.gamma(i)
+                    if T > 0:
+                        Z = np.exp(-cnst.h * E / (cnst.k * T))
+                    else:
+                        Z = np.where(E == np.min(E), 1.0, 0.0)
+                    Z = Z / np.sum(Z) if np.sum(Z) > 0 else np.ones(len(E)) / len(E)
+                    r = DensityOperator((V * Z) @ V.conj().T)
+                rho = r if rho is None else rho.kron(r)
+            self._rho0 = rho
+        return self._rho0
+
+    @property
+    def Hsys(self):
+        if self._Hsys is None:
+            self._Hsys = self._system.hamiltonian
+        return self._Hsys
+
+    @property
+    def Hz(self):
+        """Zeeman Hamiltonian sum_i gamma_i B.S_i (experiment.py:238-269),
+        built on the GPU."""
+        if self._Hz is None:
+            from . import gpu
+            from .engine import SystemDesc
+            s = self._system
+            bare = SystemDesc(Is=list(s.Is), gammas=list(s.gammas), terms=[],
+                              muon_index=s.muon_index)
+            self._Hz = Hamiltonian(gpu.build_h(bare, self._B), dim=s.dimension)
+        return self._Hz
+
+    @property
+    def dissipation_operators(self):
+        """[(S_i.x + i S_i.y, (a/pi) Z/(1+Z)), (S_i.x - i S_i.y, (a/pi)/(1+Z))]
+        for each dissipative spin, x, y perpendicular to B
+        (experiment.py:271-325)."""
+        if self._dops is None:
+            T = self._T
+            Bn = np.linalg.norm(self._B)
+            g = self._system.gammas
+            if T > 0:
+                Zu = np.exp(-cnst.h * g * Bn * 1e6 / (cnst.k * T))
+            else:
+                Zu = g * 0.0
+            if Bn == 0:
+                x, y = np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
+            else:
+                x, y = _perp(self._B / Bn)
+            self._dops = []
+            for i, a in self._config.dissipation_terms.items():
+                ops = [self._system.operator({i: e}) for e in "xyz"]
+                ox = ops[0] * x[0] + ops[1] * x[1] + ops[2] * x[2]
+                oy = ops[0] * y[0] + ops[1] * y[1] + ops[2] * y[2]
+                self._dops.append((ox + oy * 1.0j, 1 / np.pi * a * Zu[i] / (1 + Zu[i])))
+                self._dops.append((ox - oy * 1.0j, 1 / np.pi * a / (1 + Zu[i])))
+        return self._dops
+
+    @property
+    def Htot(self):
+        H = self.Hsys + self.Hz
+        H = Hamiltonian(H.matrix, dim=H.dimension)
+        if len(self._config.dissipation_terms) > 0:
+            H = Lindbladian.from_hamiltonian(H, self.dissipation_operators)
+        return H
+
+    @property
+    def p_operator(self):
+        return self._system.muon_operator(self._p)
+
+    # ---------------------------------------------------------- snapshots
+    def load_config(self, cfg_snap):
+        """B' = q.B + B_intr, p' = q.p, T; returns the orientation weight
+        (experiment.py:384-432)."""
+        q, w = cfg_snap.orient
+        self.B = rotate(q, cfg_snap.B) + np.asarray(cfg_snap.intrinsic_B, dtype=float)
+        self.p = rotate(q, cfg_snap.mupol)
+        self.T = cfg_snap.T
+        Bn = np.linalg.norm(self.B)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            check = (cnst.e * (cnst.hbar ** 2) * Bn) / (2 * cnst.m_p * cnst.k * self.T)
+        self._T_inf_speedup = bool(check == 0) and self._system.muon_index == 0
+        return w
+
+    def run_single(self, cfg_snap):
+        """One configuration through the operator classes (experiment.py:434-498)."""
+        w = self.load_config(cfg_snap)
+        H = self.Htot
+        if cfg_snap.y == "asymmetry":
+            t = np.array(cfg_snap.t, dtype=float)
+            if self._T_inf_speedup and not isinstance(H, Lindbladian):
+                d = int(np.prod(self._system.dimension)) // self._system.dimension[
+                    self._system.muon_index]
+                data = H.fast_evolve(self._system.sigma_mu(self.p), t, d)
+            else:
+                data = H.evolve(self.rho0, t, operators=[self.p_operator])[:, 0]
+        else:
+            data = H.integrate_decaying(self.rho0, MU_TAU, operators=[self.p_operator])[0] / MU_TAU
+        return np.real(data) * w
+
+    # ---------------------------------------------------------- batched run
+    def instance_batch(self):
+        """The whole configuration product as one InstanceBatch (config
+        order) plus the system description, for SpinEngine."""
+        cfg = self._config
+        snaps = cfg[0:len(cfg)]
+        n = len(snaps)
+        q = np.array([s.orient[0] for s in snaps]).reshape(n, 4)
+        w = np.array([s.orient[1] for s in snaps], dtype=float)
+        B = np.array([s.B for s in snaps], dtype=float).reshape(n, 3)
+        Bint = np.array([s.intrinsic_B for s in snaps], dtype=float).reshape(n, 3)
+        p = np.array([s.mupol for s in snaps], dtype=float).reshape(n, 3)
+        T = np.array([s.T for s in snaps], dtype=float)
+        shape = cfg.results.shape
+        time_is_x = cfg.x_axis == "t"
+        if cfg.y_axis == "integral":
+            times = np.zeros(1)
+        else:
+            times = np.atleast_1d(np.array(snaps[0].t, dtype=float))
+        if time_is_x:
+            slot = np.array([np.ravel_multi_index(tuple(s.id[0]), shape[:-1]) if shape[:-1] else 0
+                             for s in snaps], dtype=np.int64)
+            n_slots = int(np.prod(shape[:-1])) if shape[:-1] else 1
+        else:
+            slot = np.array([np.ravel_multi_index(cfg.result_index(s.id), shape) for s in snaps],
+                            dtype=np.int64)
+            n_slots = int(np.prod(shape))
+        desc = self._system.to_desc()
+        desc.dissip = dict(cfg.dissipation_terms)
+        batch = InstanceBatch(q=q, B=B, Bint=Bint, p=p, T=T, w=w, slot=slot,
+                              sys_id=np.zeros(n, dtype=np.int64), times=times, y=cfg.y_axis,
+                              n_slots=n_slots, avg_N=cfg.avg_N)
+        return [desc], batch
+
+    def run(self):
+        """Every configuration on the GPU; results (file ranges..., x), with
+        the results function applied when no fitting variables are set
+        (experiment.py:358-382)."""
+        cfg = self._config
+        systems, batch = self.instance_batch()
+        if self._engine is None:
+            self._engine = SpinEngine("cuda:0")
+        out = self._engine.run(systems, batch)          # (n_slots, width)
+        shape = cfg.results.shape
+        if cfg.x_axis == "t":
+            res = out.reshape(shape)
+        else:
+            res = out.reshape(shape + (out.shape[1],))
+            res = np.mean(res, axis=-1) if cfg.time_averaged else res[..., 0]
+        if cfg.x_axis == "t" and cfg.time_averaged:
+            raise NotImplementedError("time can not be both the x axis and averaged")
+        if not self._variables and cfg.results_function is not None:
+            res = np.array(cfg.results_function(x=cfg.x_axis_values, y=res)).reshape(res.shape)
+        cfg.results = res
+        return cfg.results

@@ -9,7 +9,6 @@ import torch
 from scipy import sparse
 
 from . import _native as N
-from .constants import MU_TAU  # noqa: F401  (re-exported for callers)
 
 _ENGINE = None
 
@@ -55,7 +54,8 @@ def eigh(M):
 
 
 def _rep(X, count):
-    return X.expand(count, -1, -1).contiguous() if X.shape[0] != count else X
+    """X (1, ...) repeated along the leading axis to (count, ...)."""
+    return X.expand((count,) + tuple(X.shape[1:])).contiguous() if X.shape[0] != count else X
 
 
 def _basis_change_dev(mats, U):

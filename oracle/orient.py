@@ -72,7 +72,7 @@ def rotate(q, v):
 
 
 def fibonacci_at_least(N):
-    F = [1, 1, 2, 3, 5, 8, 13]
+    F = [1, 1]
     while F[-1] < N:
         F.append(F[-1] + F[-2])
     return F
@@ -80,9 +80,9 @@ def fibonacci_at_least(N):
 
 def zcw_angles(N):
     """ZCW 'sphere' polar angle pairs (theta, phi), at least N of them."""
-    F = fibonacci_at_least(int(N))
+    F = fibonacci_at_least(max(int(N), 1))
     Nm = F[-1]
-    g = F[-3]
+    g = F[-3] if len(F) >= 3 else 1
     j = np.arange(Nm)
     theta = np.arccos(1.0 * (2.0 * np.mod(j / Nm, 1.0) - 1.0))
     phi = 2.0 * np.pi / 1.0 * np.mod(j * g / Nm, 1.0)

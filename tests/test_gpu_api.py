@@ -31,8 +31,10 @@ def test_hamiltonian_creation_and_diag(gpu_lib):
     H = Hamiltonian(SpinOperator.from_axes().matrix)
     E, U = H.diag()
     assert np.abs(E - [-0.5, 0.5]).max() < 1e-15
-    ref = np.array([[1.0, 1.0], [-1.0, 1.0]]) / 2 ** 0.5
-    assert np.abs(np.abs(U @ ref) - np.eye(2)).max() < 1e-14
+    # columns are (1, -1)/sqrt 2 and (1, 1)/sqrt 2 up to a phase each (the
+    # phase convention of LAPACK is not reproduced; observables do not see it)
+    Vexp = np.array([[1.0, 1.0], [-1.0, 1.0]]) / 2 ** 0.5
+    assert np.abs(np.abs(U.conj().T @ Vexp) - np.eye(2)).max() < 1e-14
     assert np.abs(H.basis_change(U).matrix - np.diag(E)).max() < 1e-14
 
 
@@ -108,7 +110,9 @@ def test_hamiltonian_argument_errors(gpu_lib):
     with pytest.raises(ValueError):
         H.integrate_decaying(rho, 1.0, [])
     with pytest.raises(TypeError):
-        H.fast_evolve(np.eye(2), [0.0, 1.0], 1)
+        H.integrate_decaying(sx, 1.0, sx)
+    with pytest.raises(ValueError):
+        H.fast_evolve(np.eye(2), np.array([0.0, 1.0]), 3)
 
 
 def test_spinsystem_hamiltonian_vs_oracle(gpu_lib):

@@ -31,12 +31,15 @@ def test_lindblad_inputs_match_oracle(B, T):
     rng = np.random.default_rng(3)
     Is = [0.5, 0.5, 1.0]
     g = [MU_GAMMA, ELEC_GAMMA, 3.0777]
-    s = SystemDesc(Is, g, [], 0, dissip={1: 7.0, 2: 0.5})
+    systems = [SystemDesc(Is, g, [], 0, dissip={1: 7.0, 2: 0.5}),
+               SystemDesc(Is, g, [], 0, dissip={1: 70.0, 2: 3.5})]
     batch = _batch(rng, 5, B, T)
+    batch.sys_id = np.array([0, 1, 1, 0, 1])
     idx = np.arange(5)
     Br = rotate_host(batch.q, batch.B) + batch.Bint
-    r, u, A, rates = lindblad_inputs_host([s], batch, idx, Br)
+    r, u, A, rates = lindblad_inputs_host(systems, batch, idx, Br)
     for k in idx:
+        s = systems[batch.sys_id[k]]
         Brk, prk, _ = opipe.load_config(batch.q[k], batch.B[k], batch.Bint[k], batch.p[k], T, 0)
         assert np.abs(Br[k] - Brk).max() < 1e-14
         rho = physics.rho0(Is, g, 0, Brk, prk, T)

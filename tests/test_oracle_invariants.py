@@ -166,7 +166,7 @@ def test_pipeline_rotation_field_vs_intrinsic():
     an unrotated intrinsic one for a non-trivial orientation."""
     ge, gmu = physics.ELEC_GAMMA, physics.MU_GAMMA
     s = _sys([0.5, 0.5], [ge, gmu], [("linear", 1, 1, np.array([0, 0, 1.0 / gmu]) * gmu)], 1)
-    q, _ = orient.orientation_entry(orient.zcw_angles(1)[3])
+    q, _ = orient.orientation_entry(orient.zcw_angles(13)[3])
     t = np.linspace(0, 10, 100)
     a = pipeline.run([s], _batch("asymmetry", t, [10, 0, 0], [1, 0, 0], np.inf, q=q))
     b = pipeline.run([s], _batch("asymmetry", t, [0, 0, 0], [1, 0, 0], np.inf, q=q, Bint=[10, 0, 0]))
