@@ -19,11 +19,163 @@
 namespace {
 
 constexpr int kWaves = 4;
+#ifndef MSP_TD_RB
+#define MSP_TD_RB 4     // rows per batched pass of the A22 v product (64-lane rows)
+#endif
+#ifndef MSP_TD_ROWS16
+#define MSP_TD_ROWS16 1 // n <= 192: A22 v with one row per 16-lane group
+#endif
 
 // reductions across a wave run on DPP lane moves (msp_team.h)
 __device__ inline double wave_sum(double v) { return wave_sum_all(v); }
 
-template <int NB, int QMAX>
+// Sum over the 16 lanes of each DPP row; every lane of the row gets it.
+__device__ inline double row16_sum(double v)
+{
+    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
+    return v;
+}
+
+// One pass of the 16-lane-row A22 v product over rows ib + g, with the
+// first QA column slots (QA is the pass's slot count, a compile-time
+// constant so that every load of the pass is issued before the first use).
+template <int Q16, int QA>
+__device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cdbl *A,
+                                   const cdbl *vb, cdbl *pb, cdbl *nxt, cdbl (&colacc)[Q16])
+{
+    const int i = ib + g;
+    const bool rowok = i < n;
+    const int ic = rowok ? i : n - 1;
+    const cdbl *row = A + (size_t)ic * n;
+    cdbl av[QA];
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+        const int c = k + 1 + li + 16 * q;
+        av[q] = row[(c <= ic) ? c : ic];
+    }
+    const cdbl vi = vb[ic];
+    double rr = 0.0, ri = 0.0;
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+        const int c = k + 1 + li + 16 * q;
+        const bool ok = rowok && c <= i;
+        const cdbl a = ok ? av[q] : cmk(0.0, 0.0);
+        const cdbl vc = vb[(c < n) ? c : n - 1];
+        rr += a.re * vc.re - a.im * vc.im;
+        ri += a.re * vc.im + a.im * vc.re;
+        const cdbl vm = (c < i) ? vi : cmk(0.0, 0.0);
+        cfmac(colacc[q], a, vm);
+        if (q == 0 && li == 0 && rowok) nxt[i] = av[0];         // column k+1
+    }
+    rr = row16_sum(rr);
+    ri = row16_sum(ri);
+    if (li == 0 && rowok) pb[i] = cmk(rr, ri);
+}
+
+// A22 v for step k with one matrix row per 16-lane group: wave w takes rows
+// k+1 + 16 t + 4 w + g (g = lane / 16), a lane the columns k+1 + (lane % 16)
+// + 16 q.  A row sum is a 16-lane DPP reduction shared by the wave's four
+// rows; the column sums conj(a_ic) v_i stay in registers and are combined
+// across the four groups once per step.  pb[i] receives the row sums, nxt[i]
+// the column k+1 (next step's column), cp[wave][c] the wave's column sums.
+template <int Q16>
+__device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdbl *A, const cdbl *vb,
+                                     cdbl *pb, cdbl *nxt, cdbl *cp)
+{
+    static_assert(Q16 == 4 || Q16 == 8 || Q16 == 12, "16-lane rows cover n <= 192");
+    const int g = lane >> 4, li = lane & 15;
+    cdbl colacc[Q16];
+#pragma unroll
+    for (int q = 0; q < Q16; ++q) colacc[q] = cmk(0.0, 0.0);
+    for (int ib = k + 1 + 4 * wave; ib < n; ib += 16) {
+        const int ilast = (ib + 3 < n) ? ib + 3 : n - 1;
+        const int qact = ((ilast - (k + 1)) >> 4) + 1;          // wave-uniform, <= Q16
+        // slot counts in steps of 4 keep the number of code paths small
+        if (qact <= 4) rows16_pass<Q16, 4>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        else if (Q16 >= 8 && qact <= 8) rows16_pass<Q16, (Q16 >= 8 ? 8 : 4)>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        else rows16_pass<Q16, Q16>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+    }
+    const int qstep = ((n - 1 - (k + 1)) >> 4) + 1;
+#pragma unroll
+    for (int q = 0; q < Q16; ++q) {
+        if (q < qstep) {
+            cdbl t = colacc[q];
+            t.re += __shfl_xor(t.re, 16, 64);
+            t.im += __shfl_xor(t.im, 16, 64);
+            t.re += __shfl_xor(t.re, 32, 64);
+            t.im += __shfl_xor(t.im, 32, 64);
+            const int c = k + 1 + li + 16 * q;
+            if (g == 0 && c < n) cp[wave * n + c] = t;
+        }
+    }
+}
+
+// One pass of the end-of-panel rank-2k update over rows i0 and i0 + 4 of
+// this wave, columns t0 + lane + 64 q for q < QA (compile-time, so the loads
+// of the pass are all issued before the first use):
+// A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl) for c <= i.
+template <int NB, int QMAX, int QA>
+__device__ inline void panel_pass(int n, int t0, int i0, int lane, cdbl *A, const cdbl *Vp,
+                                  const cdbl *Wp, cdbl *col)
+{
+    constexpr int LDP = NB + 1;
+    cdbl av[2][QA];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int i = i0 + u * kWaves;
+        const int ic = (i < n) ? i : n - 1;
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int c = t0 + lane + 64 * q;
+            av[u][q] = A[(size_t)ic * n + ((c <= ic) ? c : ic)];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int i = i0 + u * kWaves;
+        if (i >= n) continue;
+        cdbl vr[NB], wr[NB];
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+            vr[l] = Vp[i * LDP + l];
+            wr[l] = Wp[i * LDP + l];
+        }
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int c = t0 + lane + 64 * q;
+            if (c <= i) {
+                cdbl a = av[u][q];
+#pragma unroll
+                for (int l = 0; l < NB; ++l) {
+                    const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
+                    a = csub(a, cmul(vr[l], cconj(wc)));
+                    a = csub(a, cmul(wr[l], cconj(vc)));
+                }
+                A[(size_t)i * n + c] = a;
+                if (c == t0) col[i] = a;                  // next step's column
+            }
+        }
+    }
+}
+
+// Calls panel_pass with QA = qact (1 <= qact <= QA).
+template <int NB, int QMAX, int QA>
+__device__ inline void panel_dispatch(int qact, int n, int t0, int i0, int lane, cdbl *A,
+                                      const cdbl *Vp, const cdbl *Wp, cdbl *col)
+{
+    if constexpr (QA > 1) {
+        if (qact < QA) {
+            panel_dispatch<NB, QMAX, QA - 1>(qact, n, t0, i0, lane, A, Vp, Wp, col);
+            return;
+        }
+    }
+    panel_pass<NB, QMAX, QA>(n, t0, i0, lane, A, Vp, Wp, col);
+}
+
+template <int NB, int QMAX, int RB, int Q16>
 __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E, double *work)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -116,64 +268,68 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                     if (lane == 0) y[l] = cmk(sr, si);
                 }
                 // (d) A22 v over the lower triangle of rows/cols k+1..n-1
-                cdbl colacc[QMAX];
-#pragma unroll
-                for (int q = 0; q < QMAX; ++q) colacc[q] = cmk(0.0, 0.0);
-                // four rows per pass (i, i+4, i+8, i+12 of this wave): every
-                // load of the pass is issued first, from clamped addresses with
-                // the value masked, so no branch separates the loads
-                for (int i0 = k + 1 + wave; i0 < n; i0 += 4 * kWaves) {
-                    cdbl av[4][QMAX];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * kWaves;
-                        const int ic = (i < n) ? i : n - 1;
-                        const cdbl *row = A + (size_t)ic * n;
-#pragma unroll
-                        for (int q = 0; q < QMAX; ++q) {
-                            const int c = k + 1 + lane + 64 * q;
-                            const int cc = (c <= ic) ? c : ic;
-                            av[u][q] = row[cc];
+                if constexpr (Q16 > 0) {
+                    matvec_rows16<Q16>(n, k, wave, lane, A, vb, pb, nxt, cp);
+                } else {
+                    cdbl colacc[QMAX];
+    #pragma unroll
+                    for (int q = 0; q < QMAX; ++q) colacc[q] = cmk(0.0, 0.0);
+                    // RB rows per pass (i, i+4, ... of this wave): every load of
+                    // the pass is issued first, from clamped addresses with the
+                    // value masked, so no branch separates the loads
+                    for (int i0 = k + 1 + wave; i0 < n; i0 += RB * kWaves) {
+                        cdbl av[RB][QMAX];
+    #pragma unroll
+                        for (int u = 0; u < RB; ++u) {
+                            const int i = i0 + u * kWaves;
+                            const int ic = (i < n) ? i : n - 1;
+                            const cdbl *row = A + (size_t)ic * n;
+    #pragma unroll
+                            for (int q = 0; q < QMAX; ++q) {
+                                const int c = k + 1 + lane + 64 * q;
+                                const int cc = (c <= ic) ? c : ic;
+                                av[u][q] = row[cc];
+                            }
                         }
-                    }
-                    double rr[4], ri[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int i = i0 + u * kWaves;
-                        const bool rowok = i < n;
-                        const cdbl vi = vb[rowok ? i : n - 1];
-                        rr[u] = 0.0;
-                        ri[u] = 0.0;
-#pragma unroll
-                        for (int q = 0; q < QMAX; ++q) {
-                            const int c = k + 1 + lane + 64 * q;
-                            if (rowok && c <= i) {
-                                const cdbl a = av[u][q];
-                                if (q == 0 && lane == 0) nxt[i] = a;   // column k+1
-                                const cdbl vc = vb[c];
-                                rr[u] += a.re * vc.re - a.im * vc.im;
-                                ri[u] += a.re * vc.im + a.im * vc.re;
-                                if (c < i) cfmac(colacc[q], a, vi);
+                        double rr[RB], ri[RB];
+    #pragma unroll
+                        for (int u = 0; u < RB; ++u) {
+                            const int i = i0 + u * kWaves;
+                            const bool rowok = i < n;
+                            const cdbl vi = vb[rowok ? i : n - 1];
+                            rr[u] = 0.0;
+                            ri[u] = 0.0;
+    #pragma unroll
+                            for (int q = 0; q < QMAX; ++q) {
+                                const int c = k + 1 + lane + 64 * q;
+                                if (rowok && c <= i) {
+                                    const cdbl a = av[u][q];
+                                    if (q == 0 && lane == 0) nxt[i] = a;   // column k+1
+                                    const cdbl vc = vb[c];
+                                    rr[u] += a.re * vc.re - a.im * vc.im;
+                                    ri[u] += a.re * vc.im + a.im * vc.re;
+                                    if (c < i) cfmac(colacc[q], a, vi);
+                                }
+                            }
+                        }
+    #pragma unroll
+                        for (int u = 0; u < RB; ++u) {
+                            rr[u] = wave_sum_to63(rr[u]);
+                            ri[u] = wave_sum_to63(ri[u]);
+                        }
+                        if (lane == 63) {
+    #pragma unroll
+                            for (int u = 0; u < RB; ++u) {
+                                const int i = i0 + u * kWaves;
+                                if (i < n) pb[i] = cmk(rr[u], ri[u]);
                             }
                         }
                     }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        rr[u] = wave_sum_to63(rr[u]);
-                        ri[u] = wave_sum_to63(ri[u]);
+    #pragma unroll
+                    for (int q = 0; q < QMAX; ++q) {
+                        const int c = k + 1 + lane + 64 * q;
+                        if (c < n) cp[wave * n + c] = colacc[q];
                     }
-                    if (lane == 63) {
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int i = i0 + u * kWaves;
-                            if (i < n) pb[i] = cmk(rr[u], ri[u]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < QMAX; ++q) {
-                    const int c = k + 1 + lane + 64 * q;
-                    if (c < n) cp[wave * n + c] = colacc[q];
                 }
                 __syncthreads();
                 // (e) p = tau (A22 v - Vp y1 - Wp y2), and p^H v
@@ -229,44 +385,9 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
         }
         const int t0 = k0 + nbk;
         for (int i0 = t0 + wave; i0 < n; i0 += 2 * kWaves) {
-            cdbl av[2][QMAX];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int i = i0 + u * kWaves;
-                const int ic = (i < n) ? i : n - 1;
-#pragma unroll
-                for (int q = 0; q < QMAX; ++q) {
-                    const int c = t0 + lane + 64 * q;
-                    const int cc = (c <= ic) ? c : ic;
-                    av[u][q] = A[(size_t)ic * n + cc];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int i = i0 + u * kWaves;
-                if (i >= n) continue;
-                cdbl vr[NB], wr[NB];
-#pragma unroll
-                for (int l = 0; l < NB; ++l) {
-                    vr[l] = Vp[i * LDP + l];
-                    wr[l] = Wp[i * LDP + l];
-                }
-#pragma unroll
-                for (int q = 0; q < QMAX; ++q) {
-                    const int c = t0 + lane + 64 * q;
-                    if (c <= i) {
-                        cdbl a = av[u][q];
-#pragma unroll
-                        for (int l = 0; l < NB; ++l) {
-                            const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
-                            a = csub(a, cmul(vr[l], cconj(wc)));
-                            a = csub(a, cmul(wr[l], cconj(vc)));
-                        }
-                        A[(size_t)i * n + c] = a;
-                        if (c == t0) col[i] = a;                  // next step's column
-                    }
-                }
-            }
+            const int ilast = (i0 + kWaves < n) ? i0 + kWaves : n - 1;
+            const int qact = ((ilast - t0) >> 6) + 1;           // wave-uniform, <= QMAX
+            panel_dispatch<NB, QMAX, QMAX>(qact, n, t0, i0, lane, A, Vp, Wp, col);
         }
         __syncthreads();
     }
@@ -280,14 +401,15 @@ size_t lp_lds(int n)
            16 * sizeof(double);
 }
 
-template <int NB, int QMAX>
+template <int NB, int QMAX, int RB, int Q16>
 hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
     size_t lds = lp_lds<NB, QMAX>(n);
-    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX>,
+    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX, RB, Q16>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX>), dim3(batch), dim3(256), lds, s, n, H, E, work);
+    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX, RB, Q16>), dim3(batch), dim3(256), lds, s, n, H, E,
+                       work);
     return hipGetLastError();
 }
 
@@ -296,10 +418,10 @@ hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipSt
 // Lower-triangle panel kernel for n <= 512; returns hipErrorNotSupported above.
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
-    if (n <= 64) return lp_launch<8, 1>(n, batch, H, E, work, s);
-    if (n <= 128) return lp_launch<8, 2>(n, batch, H, E, work, s);
-    if (n <= 192) return lp_launch<8, 3>(n, batch, H, E, work, s);
-    if (n <= 256) return lp_launch<8, 4>(n, batch, H, E, work, s);
-    if (n <= 512) return lp_launch<4, 8>(n, batch, H, E, work, s);
+    if (n <= 64) return lp_launch<8, 1, MSP_TD_RB, MSP_TD_ROWS16 ? 4 : 0>(n, batch, H, E, work, s);
+    if (n <= 128) return lp_launch<8, 2, MSP_TD_RB, MSP_TD_ROWS16 ? 8 : 0>(n, batch, H, E, work, s);
+    if (n <= 192) return lp_launch<8, 3, MSP_TD_RB, MSP_TD_ROWS16 ? 12 : 0>(n, batch, H, E, work, s);
+    if (n <= 256) return lp_launch<8, 4, 4, 0>(n, batch, H, E, work, s);
+    if (n <= 512) return lp_launch<4, 8, 4, 0>(n, batch, H, E, work, s);
     return hipErrorNotSupported;
 }
