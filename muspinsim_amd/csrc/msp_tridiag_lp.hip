@@ -25,6 +25,9 @@ constexpr int kWaves = 4;
 #ifndef MSP_TD_ROWS16
 #define MSP_TD_ROWS16 1 // n <= 192: A22 v with one row per 16-lane group
 #endif
+#ifndef MSP_TD_RP
+#define MSP_TD_RP 1     // rows per 16-lane group in one pass of that product
+#endif
 
 // reductions across a wave run on DPP lane moves (msp_team.h)
 __device__ inline double wave_sum(double v) { return wave_sum_all(v); }
@@ -39,40 +42,47 @@ __device__ inline double row16_sum(double v)
     return v;
 }
 
-// One pass of the 16-lane-row A22 v product over rows ib + g, with the
-// first QA column slots (QA is the pass's slot count, a compile-time
-// constant so that every load of the pass is issued before the first use).
-template <int Q16, int QA>
+// One pass of the 16-lane-row A22 v product over rows ib + g + 16 r
+// (r < RP), with the first QA column slots (QA is a compile-time constant so
+// that every load of the pass is issued before the first use).
+template <int Q16, int QA, int RP>
 __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cdbl *A,
                                    const cdbl *vb, cdbl *pb, cdbl *nxt, cdbl (&colacc)[Q16])
 {
-    const int i = ib + g;
-    const bool rowok = i < n;
-    const int ic = rowok ? i : n - 1;
-    const cdbl *row = A + (size_t)ic * n;
-    cdbl av[QA];
+    cdbl av[RP][QA];
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-        const int c = k + 1 + li + 16 * q;
-        av[q] = row[(c <= ic) ? c : ic];
-    }
-    const cdbl vi = vb[ic];
-    double rr = 0.0, ri = 0.0;
+    for (int r = 0; r < RP; ++r) {
+        const int i = ib + g + 16 * r;
+        const int ic = (i < n) ? i : n - 1;
+        const cdbl *row = A + (size_t)ic * n;
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-        const int c = k + 1 + li + 16 * q;
-        const bool ok = rowok && c <= i;
-        const cdbl a = ok ? av[q] : cmk(0.0, 0.0);
-        const cdbl vc = vb[(c < n) ? c : n - 1];
-        rr += a.re * vc.re - a.im * vc.im;
-        ri += a.re * vc.im + a.im * vc.re;
-        const cdbl vm = (c < i) ? vi : cmk(0.0, 0.0);
-        cfmac(colacc[q], a, vm);
-        if (q == 0 && li == 0 && rowok) nxt[i] = av[0];         // column k+1
+        for (int q = 0; q < QA; ++q) {
+            const int c = k + 1 + li + 16 * q;
+            av[r][q] = row[(c <= ic) ? c : ic];
+        }
     }
-    rr = row16_sum(rr);
-    ri = row16_sum(ri);
-    if (li == 0 && rowok) pb[i] = cmk(rr, ri);
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+        const int i = ib + g + 16 * r;
+        const bool rowok = i < n;
+        const cdbl vi = vb[rowok ? i : n - 1];
+        double rr = 0.0, ri = 0.0;
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int c = k + 1 + li + 16 * q;
+            const bool ok = rowok && c <= i;
+            const cdbl a = ok ? av[r][q] : cmk(0.0, 0.0);
+            const cdbl vc = vb[(c < n) ? c : n - 1];
+            rr += a.re * vc.re - a.im * vc.im;
+            ri += a.re * vc.im + a.im * vc.re;
+            const cdbl vm = (c < i) ? vi : cmk(0.0, 0.0);
+            cfmac(colacc[q], a, vm);
+            if (q == 0 && li == 0 && rowok) nxt[i] = av[r][0];   // column k+1
+        }
+        rr = row16_sum(rr);
+        ri = row16_sum(ri);
+        if (li == 0 && rowok) pb[i] = cmk(rr, ri);
+    }
 }
 
 // A22 v for step k with one matrix row per 16-lane group: wave w takes rows
@@ -90,13 +100,16 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
     cdbl colacc[Q16];
 #pragma unroll
     for (int q = 0; q < Q16; ++q) colacc[q] = cmk(0.0, 0.0);
-    for (int ib = k + 1 + 4 * wave; ib < n; ib += 16) {
-        const int ilast = (ib + 3 < n) ? ib + 3 : n - 1;
+    constexpr int RP = MSP_TD_RP;
+    for (int ib = k + 1 + 4 * wave; ib < n; ib += 16 * RP) {
+        const int ilast0 = ib + 16 * (RP - 1) + 3;
+        const int ilast = (ilast0 < n) ? ilast0 : n - 1;
         const int qact = ((ilast - (k + 1)) >> 4) + 1;          // wave-uniform, <= Q16
         // slot counts in steps of 4 keep the number of code paths small
-        if (qact <= 4) rows16_pass<Q16, 4>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
-        else if (Q16 >= 8 && qact <= 8) rows16_pass<Q16, (Q16 >= 8 ? 8 : 4)>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
-        else rows16_pass<Q16, Q16>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        if (qact <= 4) rows16_pass<Q16, 4, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        else if (Q16 >= 8 && qact <= 8)
+            rows16_pass<Q16, (Q16 >= 8 ? 8 : 4), RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        else rows16_pass<Q16, Q16, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
     }
     const int qstep = ((n - 1 - (k + 1)) >> 4) + 1;
 #pragma unroll
