@@ -39,6 +39,10 @@
  *   msp_mode_sum        Lindbladian.evolve expectation values, lindbladian.py:99-103
  *   msp_mode_integral   Lindbladian.integrate_decaying, lindbladian.py:163-171
  *   msp_mode_states     Lindbladian.evolve density matrices, lindbladian.py:105-109
+ *   msp_adjoint_freq, msp_hadamard_real, msp_adjoint_gprime, msp_batch_sum,
+ *   msp_trace_products  adjoint gradient of the fast-path polarisation for fitting; no
+ *                       counterpart (FittingRunner differentiates by finite differences,
+ *                       fitting.py:99-105)
  *   msp_accumulate      Re(data)*w and store_time_slice powder accumulation,
  *                       experiment.py:498; simconfig.py:347-368
  */
@@ -190,6 +194,33 @@ int msp_mode_integral(int N, int batch, int n_ops, const double *d_lam, const do
 /* One instance: d_out[t] = V (c exp(2 pi lambda t)) (N complex per time, ntimes <= 65535). */
 int msp_mode_states(int N, int ntimes, const double *d_lam, const double *d_c, const double *d_V,
                     const double *d_times, double *d_out, void *stream);
+
+/* Adjoint of the fast-path time sum P_b(t) = (1/4d) sum_jk |O_jk|^2 cos(w_jk t),
+ * w_jk = 2 pi (E_j - E_k), for upstream weights r_b(t) = scale w_b r[slot_b][t]:
+ *   d_R[b]_jk = sum_t r_b(t) cos(w_jk t),  d_S[b]_jk = sum_t r_b(t) t sin(w_jk t)
+ * (real n x n each; d_slot, d_w may be NULL for slot 0 / weight 1; batch <= 65535). */
+int msp_adjoint_freq(int n, int batch, int ntimes, const double *d_E, const double *d_times,
+                     const double *d_r, const int32_t *d_slot, const double *d_w, double scale,
+                     double *d_R, double *d_S, void *stream);
+
+/* d_M = d_R o d_O elementwise (R real, O and M complex). */
+int msp_hadamard_real(int n, int batch, const double *d_R, const double *d_O, double *d_M,
+                      void *stream);
+
+/* Eigenbasis gradient G' (Hermitian part) of L = sum r P:
+ *   G'_jj = -(pi/d) sum_k |O_jk|^2 S_jk
+ *   G'_jm = (1/2d) (MO - OM)_jm / (E_j - E_m), or, when |E_j - E_m| <= degtol max|E|,
+ *           (1/2d) sum_k O_jk O_km (-2 pi S_jk)  (the degenerate limit). */
+int msp_adjoint_gprime(int n, int batch, const double *d_E, const double *d_O, const double *d_MO,
+                       const double *d_OM, const double *d_S, double d, double degtol,
+                       double *d_G, void *stream);
+
+/* d_out[i] += sum_b d_X[b][i] over complex entries (len per instance), in batch order. */
+int msp_batch_sum(int len, int batch, const double *d_X, double *d_out, void *stream);
+
+/* d_out[p] = Re sum_ab G_ab dH_p[b][a] for nparam complex n x n matrices dH_p. */
+int msp_trace_products(int n, int nparam, const double *d_G, const double *d_dH, double *d_out,
+                       void *stream);
 
 /* results[s][t] += rows[i][t] * scale for i in d_idx[d_ptr[s] .. d_ptr[s+1]),
  * in that order (deterministic powder accumulation). */

@@ -8,6 +8,9 @@ import ctypes
 import os
 
 import numpy as np
+# torch's HIP runtime must be the one in the process before the library is
+# loaded: both resolve libamdhip64 by soname, and the first one loaded wins.
+import torch  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmuspin_hip.so")
@@ -63,6 +66,11 @@ SIGNATURES = {
     "msp_mode_sum": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P]),
     "msp_mode_integral": (_I, [_I, _I, _I, _P, _P, _D, _D, _P, _I, _P, _P]),
     "msp_mode_states": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
+    "msp_adjoint_freq": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
+    "msp_hadamard_real": (_I, [_I, _I, _P, _P, _P, _P]),
+    "msp_adjoint_gprime": (_I, [_I, _I, _P, _P, _P, _P, _P, _D, _D, _P, _P]),
+    "msp_batch_sum": (_I, [_I, _I, _P, _P, _P]),
+    "msp_trace_products": (_I, [_I, _I, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
     "msp_profile_enable": (_I, [_I]),
     "msp_profile_report": (_I, [ctypes.c_char_p, _SZ]),

@@ -307,6 +307,66 @@ int msp_mode_states(int N, int ntimes, const double *d_lam, const double *d_c, c
                       "msp_mode_states");
 }
 
+int msp_adjoint_freq(int n, int batch, int ntimes, const double *d_E, const double *d_times,
+                     const double *d_r, const int32_t *d_slot, const double *d_w, double scale,
+                     double *d_R, double *d_S, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_adjoint_freq: n out of range");
+    if (batch < 0 || batch > 65535 || ntimes < 0)
+        return fail(MSP_ERR_VALUE, "msp_adjoint_freq: batch (<= 65535) or ntimes out of range");
+    if (batch == 0) return MSP_OK;
+    if (!d_E || !d_R || !d_S || (ntimes > 0 && (!d_times || !d_r)))
+        return fail(MSP_ERR_VALUE, "msp_adjoint_freq: null pointer");
+    return hip_status(launch_adjoint_freq(n, batch, ntimes, d_E, d_times, d_r, d_slot, d_w, scale,
+                                          d_R, d_S, S(stream)),
+                      "msp_adjoint_freq");
+}
+
+int msp_hadamard_real(int n, int batch, const double *d_R, const double *d_O, double *d_M,
+                      void *stream)
+{
+    if (n < 1 || n > kMaxDim || batch < 0)
+        return fail(MSP_ERR_VALUE, "msp_hadamard_real: size out of range");
+    if (batch == 0) return MSP_OK;
+    if (!d_R || !d_O || !d_M) return fail(MSP_ERR_VALUE, "msp_hadamard_real: null pointer");
+    return hip_status(launch_hadamard_real(n, batch, d_R, d_O, d_M, S(stream)),
+                      "msp_hadamard_real");
+}
+
+int msp_adjoint_gprime(int n, int batch, const double *d_E, const double *d_O, const double *d_MO,
+                       const double *d_OM, const double *d_S, double d, double degtol,
+                       double *d_G, void *stream)
+{
+    if (n < 1 || n > kMaxDim || batch < 0)
+        return fail(MSP_ERR_VALUE, "msp_adjoint_gprime: size out of range");
+    if (!(d > 0.0)) return fail(MSP_ERR_VALUE, "msp_adjoint_gprime: d must be > 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_E || !d_O || !d_MO || !d_OM || !d_S || !d_G)
+        return fail(MSP_ERR_VALUE, "msp_adjoint_gprime: null pointer");
+    return hip_status(launch_adjoint_gprime(n, batch, d_E, d_O, d_MO, d_OM, d_S, d, degtol, d_G,
+                                            S(stream)),
+                      "msp_adjoint_gprime");
+}
+
+int msp_batch_sum(int len, int batch, const double *d_X, double *d_out, void *stream)
+{
+    if (len < 0 || batch < 0) return fail(MSP_ERR_VALUE, "msp_batch_sum: negative size");
+    if (len == 0 || batch == 0) return MSP_OK;
+    if (!d_X || !d_out) return fail(MSP_ERR_VALUE, "msp_batch_sum: null pointer");
+    return hip_status(launch_batch_sum(len, batch, d_X, d_out, S(stream)), "msp_batch_sum");
+}
+
+int msp_trace_products(int n, int nparam, const double *d_G, const double *d_dH, double *d_out,
+                       void *stream)
+{
+    if (n < 1 || n > kMaxDim || nparam < 0)
+        return fail(MSP_ERR_VALUE, "msp_trace_products: size out of range");
+    if (nparam == 0) return MSP_OK;
+    if (!d_G || !d_dH || !d_out) return fail(MSP_ERR_VALUE, "msp_trace_products: null pointer");
+    return hip_status(launch_trace_products(n, nparam, d_G, d_dH, d_out, S(stream)),
+                      "msp_trace_products");
+}
+
 int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
                    const double *d_rows, double scale, double *d_results, void *stream)
 {

@@ -74,3 +74,16 @@ hipError_t launch_mode_integral(int N, int batch, int n_ops, const double *lam, 
                                 double *out, hipStream_t s);
 hipError_t launch_mode_states(int N, int ntimes, const double *lam, const double *c,
                               const double *V, const double *times, double *out, hipStream_t s);
+
+// Adjoint of the fast-path time sum (msp_adjoint.hip)
+hipError_t launch_adjoint_freq(int n, int batch, int ntimes, const double *E, const double *times,
+                               const double *r, const int32_t *slot, const double *w, double scale,
+                               double *R, double *S, hipStream_t s);
+hipError_t launch_hadamard_real(int n, int batch, const double *R, const double *O, double *M,
+                                hipStream_t s);
+hipError_t launch_adjoint_gprime(int n, int batch, const double *E, const double *O,
+                                 const double *MO, const double *OM, const double *S, double d,
+                                 double degtol, double *G, hipStream_t s);
+hipError_t launch_batch_sum(int len, int batch, const double *X, double *out, hipStream_t s);
+hipError_t launch_trace_products(int n, int nparam, const double *G, const double *dH, double *out,
+                                 hipStream_t s);

@@ -222,22 +222,50 @@ def c4_dissipative(seed=44, ninst=1024, ntimes=2000, B=(0.0, 0.0, 0.0), T=np.inf
 def c5_system(params=None, seed=5):
     """mu, e, 4 x 1H; params = 6 hyperfine components (xx, yy, zz, xy, xz, yz)
     and 4 muon-proton distances (Angstrom)."""
-    rng = np.random.default_rng(seed)
-    if params is None:
-        params = np.concatenate([[300.0, 320.0, 280.0, 10.0, -5.0, 8.0], rng.uniform(1.2, 2.5, 4)])
+    params, dirs = _c5_defaults(params, seed)
     a = params[:6]
     A = np.array([[a[0], a[3], a[4]], [a[3], a[1], a[5]], [a[4], a[5], a[2]]])
     Is = [0.5] * 6
     g = [MU_GAMMA, ELEC_GAMMA] + [G_H] * 4
     terms = [("bilinear", 0, 1, A)]
-    dirs = rng.normal(size=(4, 3))
-    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
     for k in range(4):
         r = dirs[k] * params[6 + k]
         terms.append(("bilinear", 0, 2 + k, dipolar_tensor(g[0], g[2 + k], r)))
         # fixed isotropic proton-electron hyperfine (not a fit parameter)
         terms.append(("bilinear", 2 + k, 1, 10.0 * np.eye(3)))
     return SystemDesc(Is, g, terms, 0), params
+
+
+def _c5_defaults(params, seed):
+    """Default parameters and the (parameter-independent) proton directions."""
+    rng = np.random.default_rng(seed)
+    default = np.concatenate([[300.0, 320.0, 280.0, 10.0, -5.0, 8.0], rng.uniform(1.2, 2.5, 4)])
+    dirs = rng.normal(size=(4, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    return (default if params is None else np.asarray(params, dtype=float)), dirs
+
+
+C5_PARAMS = ("A_xx", "A_yy", "A_zz", "A_xy", "A_xz", "A_yz", "r_1", "r_2", "r_3", "r_4")
+
+
+def c5_dterms(params, seed=5):
+    """Derivative term lists dT/dtheta_p of the c5 system, one per parameter:
+    unit symmetric tensors for the hyperfine components and -3/r_k times the
+    muon-proton dipolar tensor for each distance (D ~ r^-3 at a fixed
+    direction)."""
+    params, dirs = _c5_defaults(params, seed)
+    out = []
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for a, b in pairs:
+        E = np.zeros((3, 3))
+        E[a, b] = 1.0
+        E[b, a] = 1.0
+        out.append([("bilinear", 0, 1, E)])
+    for k in range(4):
+        r = params[6 + k]
+        D = dipolar_tensor(MU_GAMMA, G_H, dirs[k] * r)
+        out.append([("bilinear", 0, 2 + k, -3.0 / r * D)])
+    return out
 
 
 def c5_fit(seed=5, nori=256, ntimes=2000):
