@@ -2,12 +2,17 @@
 // msp_tridiag.h (U <- (I - V T V^H) U for reflector blocks of MSP_BT_NB = 16,
 // last block first) with both block products on v_mfma_f64_16x16x4f64.
 //
-// One workgroup (4 waves) owns a 16-column chunk of U in LDS (split real and
-// imaginary planes).  Per reflector block:
-//   W  = V^H U      16 x 16, the k (row) dimension split over the 4 waves,
-//                   partial tiles summed through LDS
-//   W2 = T W        16 x 16, VALU
-//   U -= V W2       16-row tiles distributed over the waves, k = 16
+// One workgroup (4 waves) owns a 16-column chunk of U for the whole
+// back-transformation, held in registers as MFMA accumulators: wave w keeps
+// the 16 x 16 row tiles w, w + 4, w + 8, ... (real and imaginary planes).
+// The accumulator of a tile is also the B operand of W = V^H U, because the
+// accumulator slot r of lane l holds U[4 r + l/16][l%16] -- the rows of the
+// k-step r of that tile.  Per reflector block:
+//   W  = V^H U      16 x 16, partial per wave over its tiles, summed in LDS
+//   W2 = T W        16 x 16, VALU, broadcast through LDS
+//   U -= V W2       in the accumulators of every tile below the block
+// U never leaves registers, so LDS holds only the W partials and W2, and
+// several workgroups share a CU.
 #include <hip/hip_runtime.h>
 
 #include "msp_kernels.h"
@@ -17,135 +22,144 @@
 namespace {
 
 constexpr int CW = 16;           // columns per workgroup
-constexpr int LDU = CW + 1;      // padded LDS row stride (doubles)
 static_assert(MSP_BT_NB == 16, "the MFMA back-transformation tiles reflector blocks of 16");
 
-__global__ void __launch_bounds__(256) k_backtransform_mfma(int n, const double *H, double *work,
-                                                            double *U)
+// Reflector j of block k0 at row r (unit at r = k0 + j + 1, stored tail
+// below, zero above), read from a clamped address so that no branch
+// separates the loads of a k-step.
+__device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+    const int k = k0 + j;
+    const bool tail = (r > k + 1) && (r < n) && (j < nbk);
+    const int rc = tail ? r : n - 1, kc = (k < n) ? k : n - 1;
+    const cdbl a = A[(size_t)rc * n + kc];
+    if (tail) return a;
+    return cmk((r == k + 1 && j < nbk) ? 1.0 : 0.0, 0.0);
+}
+
+template <int RT>
+__global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *H, double *work,
+                                                           double *U)
+{
+    __shared__ double Wp[4][2][256];          // per-wave partial W (re, im)
+    __shared__ double W2[2][256];             // T W (re, im), [row j][col c]
     const int b = blockIdx.y;
     const int c0 = blockIdx.x * CW;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lq = lane >> 4;
-    double *Ur = (double *)smem;                 // n x LDU
-    double *Ui = Ur + (size_t)n * LDU;           // n x LDU
-    double *Wp = Ui + (size_t)n * LDU;           // 4 waves x 2 x 256 partial W
-    double *W2 = Wp + 4 * 2 * 256;               // 2 x 256 (re plane, im plane)
+    const int ntiles = (n + 15) / 16;
 
     const cdbl *A = (const cdbl *)H + (size_t)b * n * n;
     double *base = work + (size_t)b * dc_work_doubles(n);
     const double *Z = dc_result_ptr(n, base);
     const cdbl *Tall = dc_tblocks(n, base);
 
-    for (int idx = tid; idx < n * CW; idx += 256) {
-        const int r = idx / CW, c = idx % CW;
-        Ur[r * LDU + c] = (c0 + c < n) ? Z[(size_t)r * n + c0 + c] : 0.0;
-        Ui[r * LDU + c] = 0.0;
+    msp_d4 ur[RT], ui[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+        const int t = wave + 4 * i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * t + 4 * r + lq, col = c0 + li;
+            ur[i][r] = (t < ntiles && row < n && col < n) ? Z[(size_t)row * n + col] : 0.0;
+            ui[i][r] = 0.0;
+        }
     }
-    __syncthreads();
 
     const int nref = n - 1;
     const int nblk = (nref + MSP_BT_NB - 1) / MSP_BT_NB;
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int k0 = blk * MSP_BT_NB;
         const int nbk = (nref - k0 < MSP_BT_NB) ? nref - k0 : MSP_BT_NB;
-        const int m = n - k0 - 1;                // rows k0+1 .. n-1
-        const int nks = (m + 3) / 4;             // k-steps of 4 rows
-        // ---- W = V^H U(k0+1.., :), partial per wave
+        // ---- W = V^H U over this wave's tiles (rows >= k0 + 1 only matter)
         msp_d4 wr = d4zero(), wi = d4zero();
-        for (int s = wave; s < nks; s += 4) {
-            const int r = k0 + 1 + 4 * s + lq;   // row of this lane's operands
-            double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
-            if (r < n) {
-                if (li < nbk) {
-                    cdbl v = vblock(n, A, k0, r, li);
-                    ar = v.re;
-                    ai = -v.im;                  // conj(V)
-                }
-                br = Ur[r * LDU + li];
-                bi = Ui[r * LDU + li];
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+            const int t = wave + 4 * i;
+            if (t < ntiles && 16 * t + 15 >= k0 + 1) {
+                cdbl v[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + 4 * s + lq, li);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    mfma_cmac(v[s].re, -v[s].im, ur[i][s], ui[i][s], wr, wi);   // conj(V)
             }
-            mfma_cmac(ar, ai, br, bi, wr, wi);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int row = 4 * q + lq;          // reflector index j
-            Wp[(wave * 2 + 0) * 256 + row * 16 + li] = wr[q];
-            Wp[(wave * 2 + 1) * 256 + row * 16 + li] = wi[q];
+            Wp[wave][0][(4 * q + lq) * 16 + li] = wr[q];
+            Wp[wave][1][(4 * q + lq) * 16 + li] = wi[q];
         }
         __syncthreads();
-        // ---- W2 = T W (VALU), one element per thread
+        // ---- W2 = T W, one element per thread
         {
             const int j = tid / 16, c = tid % 16;
             const cdbl *T = Tall + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
             double sr = 0.0, si = 0.0;
             for (int l = j; l < nbk; ++l) {
-                double wre = Wp[0 * 256 + l * 16 + c] + Wp[2 * 256 + l * 16 + c] +
-                             Wp[4 * 256 + l * 16 + c] + Wp[6 * 256 + l * 16 + c];
-                double wim = Wp[1 * 256 + l * 16 + c] + Wp[3 * 256 + l * 16 + c] +
-                             Wp[5 * 256 + l * 16 + c] + Wp[7 * 256 + l * 16 + c];
-                const cdbl t = T[j * MSP_BT_NB + l];
-                sr += t.re * wre - t.im * wim;
-                si += t.re * wim + t.im * wre;
+                const int o = l * 16 + c;
+                const double wre = Wp[0][0][o] + Wp[1][0][o] + Wp[2][0][o] + Wp[3][0][o];
+                const double wim = Wp[0][1][o] + Wp[1][1][o] + Wp[2][1][o] + Wp[3][1][o];
+                const cdbl tt = T[j * MSP_BT_NB + l];
+                sr += tt.re * wre - tt.im * wim;
+                si += tt.re * wim + tt.im * wre;
             }
-            W2[j * 16 + c] = (j < nbk) ? sr : 0.0;
-            W2[256 + j * 16 + c] = (j < nbk) ? si : 0.0;
+            W2[0][j * 16 + c] = (j < nbk) ? sr : 0.0;
+            W2[1][j * 16 + c] = (j < nbk) ? si : 0.0;
         }
         __syncthreads();
-        // ---- U(rows, :) -= V W2 on 16-row tiles
-        const int ntile = (m + 15) / 16;
-        for (int t = wave; t < ntile; t += 4) {
-            const int rt = k0 + 1 + 16 * t;
-            msp_d4 dr, di;
+        // ---- U -= V W2 in the accumulators of the tiles below the block
+        double br[4], bi[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = rt + 4 * q + lq;
-                dr[q] = (r < n) ? Ur[r * LDU + li] : 0.0;
-                di[q] = (r < n) ? Ui[r * LDU + li] : 0.0;
-            }
+        for (int s = 0; s < 4; ++s) {
+            br[s] = W2[0][(4 * s + lq) * 16 + li];
+            bi[s] = W2[1][(4 * s + lq) * 16 + li];
+        }
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int r = rt + li;           // A row (tile row li)
-                const int j = 4 * s + lq;        // A column / B row
-                double ar = 0.0, ai = 0.0;
-                if (r < n && j < nbk) {
-                    cdbl v = vblock(n, A, k0, r, j);
-                    ar = v.re;
-                    ai = v.im;
-                }
-                const double br = W2[j * 16 + li], bi = W2[256 + j * 16 + li];
-                // D -= A B
-                mfma_cmac(-ar, -ai, br, bi, dr, di);
-            }
+        for (int i = 0; i < RT; ++i) {
+            const int t = wave + 4 * i;
+            if (t < ntiles && 16 * t + 15 >= k0 + 1) {
+                cdbl v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = rt + 4 * q + lq;
-                if (r < n) {
-                    Ur[r * LDU + li] = dr[q];
-                    Ui[r * LDU + li] = di[q];
-                }
+                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + li, 4 * s + lq);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    mfma_cmac(-v[s].re, -v[s].im, br[s], bi[s], ur[i], ui[i]);
             }
         }
-        __syncthreads();
     }
     cdbl *Ub = (cdbl *)U + (size_t)b * n * n;
-    for (int idx = tid; idx < n * CW; idx += 256) {
-        const int r = idx / CW, c = idx % CW;
-        if (c0 + c < n) Ub[(size_t)r * n + c0 + c] = cmk(Ur[r * LDU + c], Ui[r * LDU + c]);
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+        const int t = wave + 4 * i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * t + 4 * r + lq, col = c0 + li;
+            if (t < ntiles && row < n && col < n)
+                Ub[(size_t)row * n + col] = cmk(ur[i][r], ui[i][r]);
+        }
     }
+}
+
+template <int RT>
+hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_backtransform_reg<RT>, dim3((n + CW - 1) / CW, batch), dim3(256), 0, s,
+                       n, H, work, U);
+    return hipGetLastError();
 }
 
 }  // namespace
 
+// n <= 512 (16-row tiles, at most 8 per wave)
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
                                      hipStream_t s)
 {
-    size_t lds = sizeof(double) * (2 * (size_t)n * LDU + 4 * 2 * 256 + 2 * 256);
-    hipError_t err = hipFuncSetAttribute((const void *)k_backtransform_mfma,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_backtransform_mfma, dim3((n + CW - 1) / CW, batch), dim3(256), lds, s, n,
-                       H, work, U);
-    return hipGetLastError();
+    const int per_wave = ((n + 15) / 16 + 3) / 4;
+    if (per_wave <= 1) return bt_launch<1>(n, batch, H, work, U, s);
+    if (per_wave <= 2) return bt_launch<2>(n, batch, H, work, U, s);
+    if (per_wave <= 3) return bt_launch<3>(n, batch, H, work, U, s);
+    if (per_wave <= 4) return bt_launch<4>(n, batch, H, work, U, s);
+    if (per_wave <= 8) return bt_launch<8>(n, batch, H, work, U, s);
+    return hipErrorNotSupported;
 }
