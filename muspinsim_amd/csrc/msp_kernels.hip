@@ -298,7 +298,7 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 __global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info,
                                                int small_in_lds)
 {
-    __shared__ double red[2 * 64 * 16];
+    __shared__ double red[256];
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = blockIdx.x;
     double *e;

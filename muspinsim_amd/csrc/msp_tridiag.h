@@ -274,8 +274,9 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
 }
 
 // Qn[lo+r][rootpos[lo+j]] = sum_i QK[lo+r][lo+i] S[lo+i][lo+j] for every merge
-// block of level L.  Large blocks go through 32 x 32 LDS tiles (the team's
-// `red` scratch must hold 2 * 32 * 33 doubles); small ones are direct dots.
+// block of level L.  Large blocks (>= 48) run on FP64 matrix cores, each
+// wave computing 32 x 32 output tiles from operands read in place; small
+// ones are direct dots.
 MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red)
 {
     // small blocks: one output per iteration, (column, row in its block) items
@@ -314,50 +315,47 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
                     Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
                 }
 #else
-            // 64 x 64 output tiles on FP64 matrix cores: K panels of 16 of QK
-            // (64 x 16) and S (16 x 64) staged in LDS (`red`, 2048 doubles);
-            // each wave owns a 32 x 32 quadrant as 2 x 2 tiles of 16 x 16.
-            const int lane = MSP_TID & 63, wave = MSP_TID >> 6;
+            // 32 x 32 output tiles (2 x 2 MFMA tiles of 16 x 16) per wave task,
+            // operands read straight from QK and S (no LDS staging, no
+            // barriers): every wave works through its tasks independently
+            const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
             const int li = lane & 15, lq = lane >> 4;
-            double *Ap = red;              // [64][16]
-            double *Bp = red + 64 * 16;    // [16][64]
-            for (int rt = 0; rt < sz; rt += 64)
-                for (int ct = 0; ct < K; ct += 64) {
-                    msp_d4 acc[2][2];
-                    for (int x = 0; x < 2; ++x)
-                        for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
-                    for (int k0 = 0; k0 < K; k0 += 16) {
-                        for (int l = MSP_TID; l < 64 * 16; l += MSP_NT) {
-                            const int r = l / 16, kk = l % 16;     // A panel
-                            const int row = rt + r, col = k0 + kk;
-                            Ap[l] = (row < sz && col < K) ? W.QK[(size_t)(blo + row) * n + blo + col] : 0.0;
-                            const int kb = l / 64, cb = l % 64;    // B panel
-                            Bp[l] = (k0 + kb < K && ct + cb < K)
-                                        ? W.S[(size_t)(blo + k0 + kb) * n + blo + ct + cb] : 0.0;
-                        }
-                        __syncthreads();
-                        const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+            const int trow = (sz + 31) / 32, tcol = (K + 31) / 32;
+            for (int task = wave; task < trow * tcol; task += nwaves) {
+                const int rt = (task / tcol) * 32, ct = (task % tcol) * 32;
+                msp_d4 acc[2][2];
+                for (int x = 0; x < 2; ++x)
+                    for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
+                for (int k0 = 0; k0 < K; k0 += 4) {
+                    const int kk = k0 + lq;
+                    const bool kok = kk < K;
+                    const int kc = kok ? kk : K - 1;
+                    double a[2], bv[2];
 #pragma unroll
-                        for (int s4 = 0; s4 < 4; ++s4) {
-                            const int kk = 4 * s4 + lq;
-                            const double a0 = Ap[(r0 + li) * 16 + kk], a1 = Ap[(r0 + 16 + li) * 16 + kk];
-                            const double b0 = Bp[kk * 64 + c0 + li], b1 = Bp[kk * 64 + c0 + 16 + li];
-                            acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
-                            acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
-                            acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
-                            acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
-                        }
-                        __syncthreads();
+                    for (int x = 0; x < 2; ++x) {
+                        const int row = rt + 16 * x + li;
+                        const int rc = (row < sz) ? row : sz - 1;
+                        const double va = W.QK[(size_t)(blo + rc) * n + blo + kc];
+                        a[x] = (row < sz && kok) ? va : 0.0;
+                        const int col = ct + 16 * x + li;
+                        const int cc = (col < K) ? col : K - 1;
+                        const double vb = W.S[(size_t)(blo + kc) * n + blo + cc];
+                        bv[x] = (col < K && kok) ? vb : 0.0;
                     }
-                    const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+#pragma unroll
                     for (int x = 0; x < 2; ++x)
-                        for (int y2 = 0; y2 < 2; ++y2)
-                            for (int q = 0; q < 4; ++q) {
-                                const int r = rt + r0 + 16 * x + 4 * q + lq;
-                                const int c = ct + c0 + 16 * y2 + li;
-                                if (r < sz && c < K) Qn[(size_t)(blo + r) * n + W.rootpos[blo + c]] = acc[x][y2][q];
-                            }
+#pragma unroll
+                        for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = mfma_f64(a[x], bv[y2], acc[x][y2]);
                 }
+                for (int x = 0; x < 2; ++x)
+                    for (int y2 = 0; y2 < 2; ++y2)
+                        for (int q = 0; q < 4; ++q) {
+                            const int r = rt + 16 * x + 4 * q + lq;
+                            const int c = ct + 16 * y2 + li;
+                            if (r < sz && c < K) Qn[(size_t)(blo + r) * n + W.rootpos[blo + c]] = acc[x][y2][q];
+                        }
+            }
+            (void)red;
 #endif
         }
         lo = (bhi > lo) ? bhi : lo + 1;
