@@ -87,3 +87,7 @@ hipError_t launch_adjoint_gprime(int n, int batch, const double *E, const double
 hipError_t launch_batch_sum(int len, int batch, const double *X, double *out, hipStream_t s);
 hipError_t launch_trace_products(int n, int nparam, const double *G, const double *dH, double *out,
                                  hipStream_t s);
+
+// Batched complex C = A^H B on FP64 matrix cores (msp_gemm_mfma.hip)
+hipError_t launch_gemm_cn_mfma(int n, int batch, const double *A, const double *B, double *C,
+                               hipStream_t s);

@@ -418,46 +418,6 @@ __global__ void k_mode_product(int n, int dk, int stride, int foff, int fstride,
     ((cdbl *)Y)[(size_t)b * n * n + idx] = acc;
 }
 
-// ---------------------------------------------------------------------------
-// C = A^H B, 32 x 32 output tile per workgroup of 256 threads, k-chunks of 16.
-__global__ void __launch_bounds__(256) k_gemm_cn(int n, const double *A, const double *B, double *C)
-{
-    __shared__ cdbl As[16][33];
-    __shared__ cdbl Bs[16][33];
-    const int b = blockIdx.z;
-    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
-    const cdbl *Ab = (const cdbl *)A + (size_t)b * n * n;
-    const cdbl *Bb = (const cdbl *)B + (size_t)b * n * n;
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    cdbl acc[2][2];
-    for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c) acc[r][c] = cmk(0.0, 0.0);
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        for (int l = threadIdx.x; l < 16 * 32; l += 256) {
-            int kk = l / 32, cc = l % 32;
-            int k = k0 + kk;
-            As[kk][cc] = (k < n && i0 + cc < n) ? Ab[(size_t)k * n + i0 + cc] : cmk(0.0, 0.0);
-            Bs[kk][cc] = (k < n && j0 + cc < n) ? Bb[(size_t)k * n + j0 + cc] : cmk(0.0, 0.0);
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; ++kk) {
-            cdbl a0 = As[kk][ty * 2], a1 = As[kk][ty * 2 + 1];
-            cdbl b0 = Bs[kk][tx * 2], b1 = Bs[kk][tx * 2 + 1];
-            cfmac(acc[0][0], a0, b0);
-            cfmac(acc[0][1], a0, b1);
-            cfmac(acc[1][0], a1, b0);
-            cfmac(acc[1][1], a1, b1);
-        }
-        __syncthreads();
-    }
-    cdbl *Cb = (cdbl *)C + (size_t)b * n * n;
-    for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c) {
-            int i = i0 + ty * 2 + r, j = j0 + tx * 2 + c;
-            if (i < n && j < n) Cb[(size_t)i * n + j] = acc[r][c];
-        }
-}
 
 // ---------------------------------------------------------------------------
 __global__ void k_form_c(int n, const double *rho, const double *obs, int real_mode,
@@ -880,10 +840,8 @@ hipError_t launch_mode_product(int n, int batch, int dk, int stride, int foff, i
 hipError_t launch_gemm_cn(int n, int batch, const double *A, const double *B, double *C,
                           hipStream_t s)
 {
-    { ProfScope ps_("k_gemm_cn", s);
-    hipLaunchKernelGGL(k_gemm_cn, dim3(cdiv(n, 32), cdiv(n, 32), batch), dim3(256), 0, s, n, A, B, C);
-    }
-    return hipGetLastError();
+    ProfScope ps_("k_gemm_cn", s);
+    return launch_gemm_cn_mfma(n, batch, A, B, C, s);
 }
 
 hipError_t launch_form_c(int n, int batch, const double *rho, const double *obs, int real_mode,

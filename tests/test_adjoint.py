@@ -159,7 +159,9 @@ def test_adjoint_fit_recovers_parameters(gpu_lib):
     target = model.forward(p_true).cpu().numpy()
     free = [0, 3, 6]
     start = p_true.copy()
-    start[free] *= np.array([1.002, 1.05, 1.01])
+    # inside the basin of the oscillatory chi^2: a 300 MHz line drifts by
+    # ~0.4 rad over 10 us at 2e-5 relative
+    start[free] *= np.array([1.00002, 1.002, 1.002])
 
     def build(th):
         p = p_true.copy()
@@ -173,5 +175,6 @@ def test_adjoint_fit_recovers_parameters(gpu_lib):
         return [d[k] for k in free]
 
     sub = PolarisationModel(build, dterms, model.batch)
-    sol = fit(sub, target, start[free], sigma=0.005)
-    assert np.abs(sol.x - p_true[free]).max() <= 1e-4 * np.abs(p_true[free]).max()
+    sol = fit(sub, target, start[free], sigma=0.005,
+              bounds=[(250.0, 350.0), (0.0, 20.0), (1.0, 4.0)])
+    assert np.all(np.abs(sol.x - p_true[free]) <= 1e-5 * np.abs(p_true[free]))
