@@ -273,11 +273,11 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
     return tau;
 }
 
-// Qn[lo+r][rootpos[lo+j]] = sum_i QK[lo+r][lo+i] S[lo+i][lo+j] for every merge
-// block of level L.  Large blocks (>= 48) run on FP64 matrix cores, each
+// Qn[lo+r][rootpos[lo+j]] = sum_i Q[lo+r][kcol[lo+i]] S[lo+i][lo+j] for every
+// merge block of level L (the kept columns are read through kcol in place).  Large blocks (>= 48) run on FP64 matrix cores, each
 // wave computing 32 x 32 output tiles from operands read in place; small
 // ones are direct dots.
-MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red)
+MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double *Qn, double *red)
 {
     // small blocks: one output per iteration, (column, row in its block) items
     const int smax = dc_maxblock(n, L);
@@ -294,7 +294,7 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
             if (j < K) {
                 double acc = 0.0;
                 for (int i = 0; i < K; ++i)
-                    acc += W.QK[(size_t)row * n + lo + i] * W.S[(size_t)(lo + i) * n + lo + j];
+                    acc += Q[(size_t)row * n + W.kcol[lo + i]] * W.S[(size_t)(lo + i) * n + lo + j];
                 Qn[(size_t)row * n + W.rootpos[lo + j]] = acc;
             }
         }
@@ -311,12 +311,12 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
                 for (int j = 0; j < K; ++j) {
                     double acc = 0.0;
                     for (int i = 0; i < K; ++i)
-                        acc += W.QK[(size_t)(blo + r) * n + blo + i] * W.S[(size_t)(blo + i) * n + blo + j];
+                        acc += Q[(size_t)(blo + r) * n + W.kcol[blo + i]] * W.S[(size_t)(blo + i) * n + blo + j];
                     Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
                 }
 #else
             // 32 x 32 output tiles (2 x 2 MFMA tiles of 16 x 16) per wave task,
-            // operands read straight from QK and S (no LDS staging, no
+            // operands read straight from Q and S (no LDS staging, no
             // barriers): every wave works through its tasks independently
             const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
             const int li = lane & 15, lq = lane >> 4;
@@ -335,7 +335,7 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, double *Qn, double *red
                     for (int x = 0; x < 2; ++x) {
                         const int row = rt + 16 * x + li;
                         const int rc = (row < sz) ? row : sz - 1;
-                        const double va = W.QK[(size_t)(blo + rc) * n + blo + kc];
+                        const double va = Q[(size_t)(blo + rc) * n + W.kcol[blo + kc]];
                         a[x] = (row < sz && kok) ? va : 0.0;
                         const int col = ct + 16 * x + li;
                         const int cc = (col < K) ? col : K - 1;
@@ -615,7 +615,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // Q is block diagonal.  Each pass runs over (column, row of the
         // column's parent block) items: rows inside the column's block get a
         // copy (deflated columns, untouched blocks) or, for kept roots, the
-        // product Q[:, kcol] S (gathered into QK, multiplied below); rows of
+        // product Q[:, kcol] S (multiplied below, Q read in place); rows of
         // the parent block outside it are set to zero, because the next
         // level's rotations and products read them.
         {
@@ -634,16 +634,15 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     } else if (hi - lo >= 2) {
                         const int src = W.outsrc[col];
                         if (src < 0) Qn[(size_t)row * n + col] = Q[(size_t)row * n + W.perm[~src]];
-                        if (col - lo < W.nkept[lo])
-                            W.QK[(size_t)row * n + col] = Q[(size_t)row * n + W.kcol[col]];
                     } else {
                         Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
                     }
                 }
             }
         }
-        MSP_SYNC();
-        dc_merge_gemm(n, L, W, Qn, red);
+        // no barrier: the products read Q and write only the kept-root
+        // columns of Qn, which the copy above leaves alone
+        dc_merge_gemm(n, L, W, Q, Qn, red);
         MSP_PFOR(s, n)
         {
             const int lo = W.blo[s], hi = W.bhi[s];
