@@ -90,6 +90,8 @@ def main():
                     help="instances of the sweep timed on the host (0 = skip)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
+    ap.add_argument("--chunk-gib", type=float, default=0.0,
+                    help="device memory budget of one chunk in GiB (0 = engine default)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -141,7 +143,12 @@ def main():
     from muspinsim_amd import _native as N
     from muspinsim_amd.engine import SpinEngine
 
-    eng = SpinEngine("cuda:0", max_chunk=args.chunk) if args.chunk else SpinEngine("cuda:0")
+    kw = {}
+    if args.chunk:
+        kw["max_chunk"] = args.chunk
+    if args.chunk_gib:
+        kw["chunk_bytes"] = int(args.chunk_gib * (1 << 30))
+    eng = SpinEngine("cuda:0", **kw)
     plan = eng.prepare(systems, batch)
     for _ in range(args.warmup):
         eng.execute(plan)

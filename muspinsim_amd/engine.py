@@ -116,7 +116,10 @@ def rotate_host(q, v):
 class SpinEngine:
     """Runs instance batches through the HIP pipeline on one device."""
 
-    def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=6 << 30,
+    # A chunk's buffers take up to 32 GiB of the 288 GiB of HBM: at n = 192
+    # that is 8192 instances, 16 full residency waves of the two-per-CU
+    # tridiagonalisation, so partial last waves and launch gaps are rare.
+    def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=32 << 30,
                  max_chunk=8192, base_bytes=1 << 30):
         self.device = torch.device(device)
         if self.device.type != "cuda":
