@@ -70,7 +70,8 @@ int hc_stedc(int n, double *d, const double *e, double *Z)
     DcAlloc al(n);
     double red[1];
     double *Q = stedc_team(n, d, e, al.W, red);
-    memcpy(Z, Q, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i)              // Q holds eigenvector columns contiguously
+        for (int j = 0; j < n; ++j) Z[(size_t)i * n + j] = Q[(size_t)j * n + i];
     return 0;
 }
 

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * t + 4 * r + lq, col = c0 + li;
-            ur[i][r] = (t < ntiles && row < n && col < n) ? Z[(size_t)row * n + col] : 0.0;
+            ur[i][r] = (t < ntiles && row < n && col < n) ? Z[(size_t)col * n + row] : 0.0;
             ui[i][r] = 0.0;
         }
     }

@@ -274,9 +274,12 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
 }
 
 // Qn[lo+r][rootpos[lo+j]] = sum_i Q[lo+r][kcol[lo+i]] S[lo+i][lo+j] for every
-// merge block of level L (the kept columns are read through kcol in place).  Large blocks (>= 48) run on FP64 matrix cores, each
-// wave computing 32 x 32 output tiles from operands read in place; small
-// ones are direct dots.
+// merge block of level L (the kept columns are read through kcol in place;
+// Q and Qn hold eigenvector columns contiguously, element (row, col) at
+// col * n + row).  Large blocks (>= 48) run on FP64 matrix cores as
+// Qn^T = S^T Q_K^T, each wave computing 32 x 32 output tiles from operands
+// read in place, every operand and result access along a contiguous 16-run;
+// small ones are direct dots.
 MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double *Qn, double *red)
 {
     // small blocks: one output per iteration, (column, row in its block) items
@@ -294,8 +297,8 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
             if (j < K) {
                 double acc = 0.0;
                 for (int i = 0; i < K; ++i)
-                    acc += Q[(size_t)row * n + W.kcol[lo + i]] * W.S[(size_t)(lo + i) * n + lo + j];
-                Qn[(size_t)row * n + W.rootpos[lo + j]] = acc;
+                    acc += Q[(size_t)W.kcol[lo + i] * n + row] * W.S[(size_t)(lo + i) * n + lo + j];
+                Qn[(size_t)W.rootpos[lo + j] * n + row] = acc;
             }
         }
     }
@@ -311,18 +314,20 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
                 for (int j = 0; j < K; ++j) {
                     double acc = 0.0;
                     for (int i = 0; i < K; ++i)
-                        acc += Q[(size_t)(blo + r) * n + W.kcol[blo + i]] * W.S[(size_t)(blo + i) * n + blo + j];
-                    Qn[(size_t)(blo + r) * n + W.rootpos[blo + j]] = acc;
+                        acc += Q[(size_t)W.kcol[blo + i] * n + blo + r] * W.S[(size_t)(blo + i) * n + blo + j];
+                    Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc;
                 }
 #else
-            // 32 x 32 output tiles (2 x 2 MFMA tiles of 16 x 16) per wave task,
-            // operands read straight from Q and S (no LDS staging, no
-            // barriers): every wave works through its tasks independently
+            // 32 x 32 tiles of Qn^T = S^T Q_K^T (2 x 2 MFMA tiles of 16 x 16)
+            // per wave task: rows j = kept roots, columns r = block rows.
+            // A operand S^T[j][k] = S[k][j] (16 consecutive j of a row of S),
+            // B operand Q_K^T[k][r] = Q(blo + r, kcol[k]) (16 consecutive r of
+            // an eigenvector column); no LDS staging and no barriers
             const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
             const int li = lane & 15, lq = lane >> 4;
-            const int trow = (sz + 31) / 32, tcol = (K + 31) / 32;
-            for (int task = wave; task < trow * tcol; task += nwaves) {
-                const int rt = (task / tcol) * 32, ct = (task % tcol) * 32;
+            const int tj = (K + 31) / 32, tr = (sz + 31) / 32;
+            for (int task = wave; task < tj * tr; task += nwaves) {
+                const int jt = (task / tr) * 32, rt = (task % tr) * 32;
                 msp_d4 acc[2][2];
                 for (int x = 0; x < 2; ++x)
                     for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
@@ -330,17 +335,19 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
                     const int kk = k0 + lq;
                     const bool kok = kk < K;
                     const int kc = kok ? kk : K - 1;
+                    const double *Qk = Q + (size_t)W.kcol[blo + kc] * n + blo;
+                    const double *Sk = W.S + (size_t)(blo + kc) * n + blo;
                     double a[2], bv[2];
 #pragma unroll
                     for (int x = 0; x < 2; ++x) {
-                        const int row = rt + 16 * x + li;
-                        const int rc = (row < sz) ? row : sz - 1;
-                        const double va = Q[(size_t)(blo + rc) * n + W.kcol[blo + kc]];
-                        a[x] = (row < sz && kok) ? va : 0.0;
-                        const int col = ct + 16 * x + li;
-                        const int cc = (col < K) ? col : K - 1;
-                        const double vb = W.S[(size_t)(blo + kc) * n + blo + cc];
-                        bv[x] = (col < K && kok) ? vb : 0.0;
+                        const int j = jt + 16 * x + li;
+                        const int jc = (j < K) ? j : K - 1;
+                        const double va = Sk[jc];
+                        a[x] = (j < K && kok) ? va : 0.0;
+                        const int r = rt + 16 * x + li;
+                        const int rc = (r < sz) ? r : sz - 1;
+                        const double vb = Qk[rc];
+                        bv[x] = (r < sz && kok) ? vb : 0.0;
                     }
 #pragma unroll
                     for (int x = 0; x < 2; ++x)
@@ -350,9 +357,9 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
                 for (int x = 0; x < 2; ++x)
                     for (int y2 = 0; y2 < 2; ++y2)
                         for (int q = 0; q < 4; ++q) {
-                            const int r = rt + 16 * x + 4 * q + lq;
-                            const int c = ct + 16 * y2 + li;
-                            if (r < sz && c < K) Qn[(size_t)(blo + r) * n + W.rootpos[blo + c]] = acc[x][y2][q];
+                            const int j = jt + 16 * x + 4 * q + lq;
+                            const int r = rt + 16 * y2 + li;
+                            if (j < K && r < sz) Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc[x][y2][q];
                         }
             }
             (void)red;
@@ -364,7 +371,8 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
 }
 
 // d (n): diagonal, overwritten by ascending eigenvalues; e (n-1): off-diagonal.
-// Q: on exit eigenvectors (points to W.Q0 or W.Q1, returned).
+// Q: on exit eigenvectors, one contiguous column each: component r of
+// eigenvector c at Q[c * n + r] (points to W.Q0 or W.Q1, returned).
 MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *red)
 {
     const double eps = 2.220446049250313e-16;
@@ -408,8 +416,8 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 int mid = lo + (hi - lo) / 2;
                 double ec = e[mid - 1] * inv;
                 double zi;
-                if (i < mid) zi = Q[(size_t)(mid - 1) * n + i];
-                else zi = (ec < 0.0 ? -1.0 : 1.0) * Q[(size_t)mid * n + i];
+                if (i < mid) zi = Q[(size_t)i * n + mid - 1];
+                else zi = (ec < 0.0 ? -1.0 : 1.0) * Q[(size_t)i * n + mid];
                 W.zv[i] = zi * 0.7071067811865476;
                 // rank within the merged (two sorted halves) order
                 double di = W.dv[i];
@@ -505,9 +513,9 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 for (int r = 0; r < nr; ++r) {
                     int a = W.rp[lo + r], b = W.rn[lo + r];
                     double c = W.rc[lo + r], s = W.rs[lo + r];
-                    double qa = Q[(size_t)row * n + a], qb = Q[(size_t)row * n + b];
-                    Q[(size_t)row * n + a] = c * qa + s * qb;
-                    Q[(size_t)row * n + b] = c * qb - s * qa;
+                    double qa = Q[(size_t)a * n + row], qb = Q[(size_t)b * n + row];
+                    Q[(size_t)a * n + row] = c * qa + s * qb;
+                    Q[(size_t)b * n + row] = c * qb - s * qa;
                 }
             }
         }
@@ -630,12 +638,12 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 const int row = plo + (idx & ((1 << sh) - 1));
                 if (row < phi) {
                     if (row < lo || row >= hi) {
-                        Qn[(size_t)row * n + col] = 0.0;
+                        Qn[(size_t)col * n + row] = 0.0;
                     } else if (hi - lo >= 2) {
                         const int src = W.outsrc[col];
-                        if (src < 0) Qn[(size_t)row * n + col] = Q[(size_t)row * n + W.perm[~src]];
+                        if (src < 0) Qn[(size_t)col * n + row] = Q[(size_t)W.perm[~src] * n + row];
                     } else {
-                        Qn[(size_t)row * n + col] = Q[(size_t)row * n + col];
+                        Qn[(size_t)col * n + row] = Q[(size_t)col * n + row];
                     }
                 }
             }
@@ -670,7 +678,7 @@ MSP_HD void backtransform_chunk_team(int n, const cdbl *A, const cdbl *tau, cons
     MSP_PFOR(idx, n * cwk)
     {
         int i = idx / cwk, c = idx % cwk;
-        Uc[i * cw + c] = cmk(Z[(size_t)i * n + c0 + c], 0.0);
+        Uc[i * cw + c] = cmk(Z[(size_t)(c0 + c) * n + i], 0.0);
     }
     MSP_SYNC();
     for (int k = n - 2; k >= 0; --k) {
@@ -834,7 +842,7 @@ MSP_HD void backtransform_blocked_team(int n, const cdbl *A, const cdbl *Tall, c
     MSP_PFOR(idx, n * cwk)
     {
         int i = idx / cwk, c = idx % cwk;
-        Uc[i * ldu + c] = cmk(Z[(size_t)i * n + c0 + c], 0.0);
+        Uc[i * ldu + c] = cmk(Z[(size_t)(c0 + c) * n + i], 0.0);
     }
     MSP_SYNC();
     const int nref = n - 1;
