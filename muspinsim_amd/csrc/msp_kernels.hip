@@ -3,6 +3,8 @@
 // team functions of msp_jacobi.h / msp_tridiag.h.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "../../include/muspin_hip.h"
 #include "msp_jacobi.h"
 #include "msp_kernels.h"
@@ -766,6 +768,20 @@ hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, doub
     return hipGetLastError();
 }
 
+// Second stream of each device, for work that overlaps the divide and conquer
+// (created on first use; default priority; nullptr if it cannot be made).
+static hipStream_t side_stream()
+{
+    static hipStream_t streams[64] = {};
+    static std::mutex mu;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+        streams[dev] = nullptr;
+    return streams[dev];
+}
+
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
                           double *work, hipStream_t s)
 {
@@ -784,6 +800,29 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     }
     if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
+    // The WY factors need only the reflectors, so they are formed on the side
+    // stream while the divide and conquer runs (it depends on d and e only);
+    // the back-transformation joins both.  Events are per call, so
+    // concurrent callers on other streams keep their own dependencies.
+    hipStream_t s2 = (bt_nblocks(n) > 0) ? side_stream() : nullptr;
+    hipEvent_t ev_td = nullptr, ev_ft = nullptr;
+    if (s2 && (hipEventCreateWithFlags(&ev_td, hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&ev_ft, hipEventDisableTiming) != hipSuccess)) {
+        if (ev_td) hipEventDestroy(ev_td);
+        ev_td = nullptr;
+        s2 = nullptr;
+    }
+    if (s2) {
+        if ((err = hipEventRecord(ev_td, s)) != hipSuccess) return err;
+        if ((err = hipStreamWaitEvent(s2, ev_td, 0)) != hipSuccess) return err;
+        {
+            ProfScope ps_("k_larft", s2);
+            hipLaunchKernelGGL(k_larft, dim3(bt_nblocks(n), batch), dim3(256), 0, s2, n,
+                               (const double *)H, work);
+        }
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        if ((err = hipEventRecord(ev_ft, s2)) != hipSuccess) return err;
+    }
     {
         const int in_lds = (n <= 512) ? 1 : 0;
         const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
@@ -794,7 +833,12 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
         hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    if (bt_nblocks(n) > 0) {
+    if (s2) {
+        err = hipStreamWaitEvent(s, ev_ft, 0);
+        hipEventDestroy(ev_td);
+        hipEventDestroy(ev_ft);
+        if (err != hipSuccess) return err;
+    } else if (bt_nblocks(n) > 0) {
         ProfScope ps_("k_larft", s);
         hipLaunchKernelGGL(k_larft, dim3(bt_nblocks(n), batch), dim3(256), 0, s, n,
                            (const double *)H, work);
