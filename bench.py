@@ -37,6 +37,10 @@ def algorithmic(kernel, n, ntimes, real_mode=True):
         return "flop", 8.0 * n ** 3
     if kernel == "k_eigh_jacobi":
         return "flop", (16.0 / 3.0 + 8.0) * n ** 3
+    if kernel == "k_lindblad_modes":
+        # Liouvillian order N = n^2: Hessenberg + Schur + eigenvectors (LAPACK's
+        # zgeev count, ~100 N^3 real flops) and the mode amplitudes (8/3 N^3)
+        return "flop", (100.0 + 8.0 / 3.0) * float(n * n) ** 3
     if kernel == "k_stedc":
         # the merge products Q_blockdiag Z: n^3 at the top level, n^3/4 below, ...
         return "flop", 4.0 / 3.0 * n ** 3
@@ -77,6 +81,10 @@ def workload(name):
         return W.c3_tf_stress(), "tf_stress_n512_256orient_4000t", "tf_stress_n512"
     if name == "c5":
         return W.c5_fit(), "fit_forward_n64_256orient_2000t", "fit_forward_n64"
+    if name == "c4":
+        return W.c4_uneven_batch(), "uneven_batch_n4_to_1024_4096inst_101t", "uneven_batch"
+    if name == "c4d":
+        return W.c4_dissipative(), "lindblad_n8_N64_1024inst_2000t", "lindblad_n8"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -111,7 +119,8 @@ def main():
 
     import numpy as np
     (systems, batch), wname, tag = workload(args.workload)
-    n = systems[0].n
+    dims = np.array([systems[int(k)].n for k in batch.sys_id])
+    n = int(dims.max())
     ninst = len(batch)
     width = 1 if batch.y == "integral" else len(batch.times)
 
@@ -123,7 +132,7 @@ def main():
         from oracle import baseline
         stride = max(1, ninst // args.cpu_sample)
         sample = np.arange(0, ninst, stride)[: args.cpu_sample]
-        procs = args.cpu_procs or min(16, baseline.host_cores())
+        procs = min(args.cpu_procs or min(16, baseline.host_cores()), len(sample))
         cpu_rows, wall, busy = baseline.run(systems, batch, sample, procs)
         one = sample[: max(1, min(len(sample), 64))]
         _, wall1, _ = baseline.run(systems, batch, one, 1)
@@ -177,7 +186,11 @@ def main():
     # dominant kernel and its roofline
     kname = max(prof, key=lambda k: prof[k][1])
     launches, tot_ms = prof[kname]
-    unit, per_inst = algorithmic(kname, n, width)
+    # algorithmic work summed over the instances (their dimensions may differ)
+    unit = algorithmic(kname, n, width)[0]
+    work = sum(algorithmic(kname, int(d), width)[1] * c
+               for d, c in zip(*np.unique(dims, return_counts=True)))
+    per_inst = work / ninst
     inst_per_launch = ninst * args.steps / launches
     avg_s = tot_ms / launches / 1e3
     if unit == "flop":
@@ -211,9 +224,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded couplings and orientations)",
-        "config": {"workload": wname, "hilbert_dim": n, "instances": ninst,
-                   "precision": "fp64-plain", "eigensolver": "householder+divide-and-conquer"
-                   if n > 64 else "jacobi-lds"},
+        "config": {"workload": wname, "hilbert_dim": n if dims.min() == n else
+                   f"{int(dims.min())}-{n}", "instances": ninst,
+                   "precision": "fp64-plain",
+                   "eigensolver": "hessenberg+complex-schur (Liouvillian)"
+                   if any(getattr(sy, "dissip", None) for sy in systems)
+                   else "householder+divide-and-conquer" if dims.min() > 64
+                   else "jacobi-lds" if n <= 64 else "jacobi-lds + householder+divide-and-conquer"},
         "max_abs_dP_vs_fp64_oracle": maxdp,
         "value_without_event_timing": ninst / dt_clean,
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(prof.items())},

@@ -32,6 +32,7 @@ def _work(args):
 def run(systems, batch, idx, procs):
     """Oracle rows for instances idx using `procs` forked workers.
     Returns (rows in idx order, wall seconds, per-worker busy seconds)."""
+    procs = max(1, min(procs, len(idx)))     # no idle workers on small samples
     _STATE.update(systems=systems, batch=batch, idx=np.asarray(idx))
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
