@@ -2,9 +2,10 @@
 // msp_tridiag.h (U <- (I - V T V^H) U for reflector blocks of MSP_BT_NB = 16,
 // last block first) with both block products on v_mfma_f64_16x16x4f64.
 //
-// One workgroup (4 waves) owns a 16-column chunk of U for the whole
-// back-transformation, held in registers as MFMA accumulators: wave w keeps
-// the 16 x 16 row tiles w, w + 4, w + 8, ... (real and imaginary planes).
+// One workgroup (NW waves: 4 up to n = 512, 8 up to n = 1024) owns a
+// 16-column chunk of U for the whole back-transformation, held in registers
+// as MFMA accumulators: wave w keeps the 16 x 16 row tiles w, w + NW, ...
+// (real and imaginary planes).
 // The accumulator of a tile is also the B operand of W = V^H U, because the
 // accumulator slot r of lane l holds U[4 r + l/16][l%16] -- the rows of the
 // k-step r of that tile.  Per reflector block:
@@ -37,11 +38,11 @@ __device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j
     return cmk((r == k + 1 && j < nbk) ? 1.0 : 0.0, 0.0);
 }
 
-template <int RT>
-__global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *H, double *work,
-                                                           double *U)
+template <int RT, int NW>
+__global__ void __launch_bounds__(64 * NW) k_backtransform_reg(int n, const double *H, double *work,
+                                                               double *U)
 {
-    __shared__ double Wp[4][2][256];          // per-wave partial W (re, im)
+    __shared__ double Wp[NW][2][256];         // per-wave partial W (re, im)
     __shared__ double W2[2][256];             // T W (re, im), [row j][col c]
     const int b = blockIdx.y;
     const int c0 = blockIdx.x * CW;
@@ -57,7 +58,7 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
     msp_d4 ur[RT], ui[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-        const int t = wave + 4 * i;
+        const int t = wave + NW * i;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * t + 4 * r + lq, col = c0 + li;
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
         msp_d4 wr = d4zero(), wi = d4zero();
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
-            const int t = wave + 4 * i;
+            const int t = wave + NW * i;
             if (t < ntiles && 16 * t + 15 >= k0 + 1) {
                 cdbl v[4];
 #pragma unroll
@@ -84,6 +85,9 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
                 for (int s = 0; s < 4; ++s)
                     mfma_cmac(v[s].re, -v[s].im, ur[i][s], ui[i][s], wr, wi);   // conj(V)
             }
+            // with 8 waves (2 per SIMD, 256 registers) keep one tile's loads
+            // in flight at a time
+            if constexpr (NW > 4) __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -91,15 +95,19 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
             Wp[wave][1][(4 * q + lq) * 16 + li] = wi[q];
         }
         __syncthreads();
-        // ---- W2 = T W, one element per thread
-        {
+        // ---- W2 = T W, one element per thread of the first 256
+        if (tid < 256) {
             const int j = tid / 16, c = tid % 16;
             const cdbl *T = Tall + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
             double sr = 0.0, si = 0.0;
             for (int l = j; l < nbk; ++l) {
                 const int o = l * 16 + c;
-                const double wre = Wp[0][0][o] + Wp[1][0][o] + Wp[2][0][o] + Wp[3][0][o];
-                const double wim = Wp[0][1][o] + Wp[1][1][o] + Wp[2][1][o] + Wp[3][1][o];
+                double wre = 0.0, wim = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < NW; ++w2) {
+                    wre += Wp[w2][0][o];
+                    wim += Wp[w2][1][o];
+                }
                 const cdbl tt = T[j * MSP_BT_NB + l];
                 sr += tt.re * wre - tt.im * wim;
                 si += tt.re * wim + tt.im * wre;
@@ -117,7 +125,7 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
         }
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
-            const int t = wave + 4 * i;
+            const int t = wave + NW * i;
             if (t < ntiles && 16 * t + 15 >= k0 + 1) {
                 cdbl v[4];
 #pragma unroll
@@ -126,12 +134,13 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
                 for (int s = 0; s < 4; ++s)
                     mfma_cmac(-v[s].re, -v[s].im, br[s], bi[s], ur[i], ui[i]);
             }
+            if constexpr (NW > 4) __builtin_amdgcn_sched_barrier(0);
         }
     }
     cdbl *Ub = (cdbl *)U + (size_t)b * n * n;
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
-        const int t = wave + 4 * i;
+        const int t = wave + NW * i;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * t + 4 * r + lq, col = c0 + li;
@@ -141,25 +150,26 @@ __global__ void __launch_bounds__(256) k_backtransform_reg(int n, const double *
     }
 }
 
-template <int RT>
+template <int RT, int NW>
 hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_backtransform_reg<RT>, dim3((n + CW - 1) / CW, batch), dim3(256), 0, s,
-                       n, H, work, U);
+    hipLaunchKernelGGL((k_backtransform_reg<RT, NW>), dim3((n + CW - 1) / CW, batch), dim3(64 * NW),
+                       0, s, n, H, work, U);
     return hipGetLastError();
 }
 
 }  // namespace
 
-// n <= 512 (16-row tiles, at most 8 per wave)
+// n <= 1024 (16-row tiles, at most 8 per wave; 8 waves above n = 512)
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
                                      hipStream_t s)
 {
+    if (n > 1024) return hipErrorNotSupported;
+    if (n > 512) return bt_launch<8, 8>(n, batch, H, work, U, s);
     const int per_wave = ((n + 15) / 16 + 3) / 4;
-    if (per_wave <= 1) return bt_launch<1>(n, batch, H, work, U, s);
-    if (per_wave <= 2) return bt_launch<2>(n, batch, H, work, U, s);
-    if (per_wave <= 3) return bt_launch<3>(n, batch, H, work, U, s);
-    if (per_wave <= 4) return bt_launch<4>(n, batch, H, work, U, s);
-    if (per_wave <= 8) return bt_launch<8>(n, batch, H, work, U, s);
-    return hipErrorNotSupported;
+    if (per_wave <= 1) return bt_launch<1, 4>(n, batch, H, work, U, s);
+    if (per_wave <= 2) return bt_launch<2, 4>(n, batch, H, work, U, s);
+    if (per_wave <= 3) return bt_launch<3, 4>(n, batch, H, work, U, s);
+    if (per_wave <= 4) return bt_launch<4, 4>(n, batch, H, work, U, s);
+    return bt_launch<8, 4>(n, batch, H, work, U, s);
 }

@@ -844,7 +844,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
                            (const double *)H, work);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    if (n <= 512) {
+    if (n <= 1024) {
         ProfScope ps_("k_backtransform", s);
         return launch_backtransform_mfma(n, batch, (const double *)H, work, U, s);
     }
