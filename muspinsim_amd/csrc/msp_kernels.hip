@@ -786,7 +786,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
                           double *work, hipStream_t s)
 {
     hipError_t err = hipSuccess;
-    if (n <= 512) {
+    if (n <= 1024) {
         ProfScope ps_("k_tridiag", s);
         err = launch_tridiag_lp(n, batch, H, E, work, s);
     } else {

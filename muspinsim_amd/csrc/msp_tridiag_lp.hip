@@ -188,16 +188,24 @@ __device__ inline void panel_dispatch(int qact, int n, int t0, int i0, int lane,
     panel_pass<NB, QMAX, QA>(n, t0, i0, lane, A, Vp, Wp, col);
 }
 
-template <int NB, int QMAX, int RB, int Q16>
+template <int NB, int QMAX, int RB, int Q16, bool GP>
 __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E, double *work)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int LDP = NB + 1;                     // padded panel row stride
-    cdbl *Vp = (cdbl *)smem;                         // n x LDP
-    cdbl *Wp = Vp + (size_t)n * LDP;                 // n x LDP
-    cdbl *vb = Wp + (size_t)n * LDP;                 // n
+    static_assert(!GP || LDP <= MSP_TD_NB, "global panels hold MSP_TD_NB columns");
+    cdbl *A = (cdbl *)H + (size_t)b * n * n;
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    double *e;
+    cdbl *tau;
+    dc_carve(n, base, &e, &tau);
+    double *d = E + (size_t)b * n;
+    // the panel pair lives in LDS, or (GP, large n) in the workspace
+    cdbl *Vp = GP ? dc_panels(n, base) : (cdbl *)smem;                 // n x LDP
+    cdbl *Wp = Vp + (size_t)n * LDP;                                     // n x LDP
+    cdbl *vb = GP ? (cdbl *)smem : Wp + (size_t)n * LDP;                // n
     cdbl *pb = vb + n;                               // n (row sums, then p)
     cdbl *cp = pb + n;                               // kWaves x n column partials
     cdbl *y = cp + (size_t)kWaves * n;               // 2 NB
@@ -205,18 +213,12 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
     cdbl *colB = colA + n;                           // n: column k+1, stashed by the matvec
     double *red = (double *)(colB + n);              // 16
 
-    cdbl *A = (cdbl *)H + (size_t)b * n * n;
-    double *base = work + (size_t)b * dc_work_doubles(n);
-    double *e;
-    cdbl *tau;
-    dc_carve(n, base, &e, &tau);
-    double *d = E + (size_t)b * n;
-
     cdbl *col = colA, *nxt = colB;
     for (int i = tid; i < n; i += 256) col[i] = A[(size_t)i * n];
     for (int k0 = 0; k0 < n - 1; k0 += NB) {
         const int nbk = (n - 1 - k0 < NB) ? n - 1 - k0 : NB;
-        for (int idx = tid; idx < n * LDP; idx += 256) {
+        // (rows above the panel are never read)
+        for (int idx = k0 * LDP + tid; idx < n * LDP; idx += 256) {
             Vp[idx] = cmk(0.0, 0.0);
             Wp[idx] = cmk(0.0, 0.0);
         }
@@ -407,28 +409,30 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
     if (tid == 0) d[n - 1] = col[n - 1].re;
 }
 
-template <int NB, int QMAX>
+template <int NB, bool GP>
 size_t lp_lds(int n)
 {
-    return sizeof(cdbl) * ((size_t)2 * n * (NB + 1) + 4 * (size_t)n + (size_t)kWaves * n + 2 * NB) +
+    return sizeof(cdbl) * ((GP ? 0 : (size_t)2 * n * (NB + 1)) + 4 * (size_t)n +
+                           (size_t)kWaves * n + 2 * NB) +
            16 * sizeof(double);
 }
 
-template <int NB, int QMAX, int RB, int Q16>
+template <int NB, int QMAX, int RB, int Q16, bool GP = false>
 hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
-    size_t lds = lp_lds<NB, QMAX>(n);
-    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX, RB, Q16>,
+    size_t lds = lp_lds<NB, GP>(n);
+    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX, RB, Q16, GP>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX, RB, Q16>), dim3(batch), dim3(256), lds, s, n, H, E,
-                       work);
+    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX, RB, Q16, GP>), dim3(batch), dim3(256), lds, s, n, H,
+                       E, work);
     return hipGetLastError();
 }
 
 }  // namespace
 
-// Lower-triangle panel kernel for n <= 512; returns hipErrorNotSupported above.
+// Lower-triangle panel kernel for n <= 1024 (panels in the workspace above
+// n = 512); returns hipErrorNotSupported above.
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
     if (n <= 64) return lp_launch<8, 1, MSP_TD_RB, MSP_TD_ROWS16 ? 4 : 0>(n, batch, H, E, work, s);
@@ -436,5 +440,6 @@ hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *wor
     if (n <= 192) return lp_launch<8, 3, MSP_TD_RB, MSP_TD_ROWS16 ? 12 : 0>(n, batch, H, E, work, s);
     if (n <= 256) return lp_launch<8, 4, 4, 0>(n, batch, H, E, work, s);
     if (n <= 512) return lp_launch<4, 8, 4, 0>(n, batch, H, E, work, s);
+    if (n <= 1024) return lp_launch<4, 16, 2, 0, true>(n, batch, H, E, work, s);
     return hipErrorNotSupported;
 }
