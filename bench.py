@@ -239,7 +239,41 @@ def main():
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "results_checksum": float(np.sum(results)),
     }
+    if args.workload == "c5":
+        line["adjoint"] = adjoint_timing(systems, batch, args.steps)
     print(json.dumps(line), flush=True)
+
+
+def adjoint_timing(systems, batch, steps):
+    """Cost of one chi^2 gradient of the fit configuration (10 parameters:
+    6 hyperfine components, 4 muon-proton distances) by the adjoint pass, next
+    to the forward pass it is built on.  The reference's L-BFGS-B takes
+    finite differences, n_params + 1 forward simulations per gradient
+    (muspinsim/fitting.py:99-105)."""
+    import numpy as np
+    import torch
+    from muspinsim_amd import workloads as W
+    from muspinsim_amd.adjoint import PolarisationModel
+    _, p0 = W.c5_system(None)
+    model = PolarisationModel(lambda th: W.c5_system(th)[0], W.c5_dterms, batch)
+    target = model.forward(p0).cpu().numpy()
+    r = np.zeros_like(target) + 1e-3
+    model.vjp(p0, r)                        # warm-up (plans, buffers)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.forward(p0)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.vjp(p0, r)
+    torch.cuda.synchronize()
+    vjp = (time.perf_counter() - t0) / steps
+    nparam = len(p0)
+    return {"params": nparam, "forward_ms": fwd * 1e3, "gradient_ms": vjp * 1e3,
+            "finite_difference_ms": (nparam + 1) * fwd * 1e3,
+            "gradient_speedup_vs_finite_differences": (nparam + 1) * fwd / vjp}
 
 
 if __name__ == "__main__":
