@@ -46,7 +46,7 @@ struct DcAlloc {
     DcWork W;
     // Workspace starts as NaN (doubles) / -1 (ints), as reused device memory
     // would hold stale values: every read must follow a write.
-    DcAlloc(int n) : dbl((size_t)4 * n * n + 12 * (size_t)n, NAN), in(20 * (size_t)n, -1)
+    DcAlloc(int n) : dbl((size_t)4 * n * n + 12 * (size_t)n, NAN), in(24 * (size_t)n, -1)
     {
         double *p = dbl.data();
         W.Q0 = p; p += (size_t)n * n;
@@ -61,6 +61,7 @@ struct DcAlloc {
         W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
         W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
         W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
+        W.sup = q; q += n; W.kperm = q; q += n; W.nka = q; q += n; W.kb0 = q; q += n;
     }
 };
 

@@ -316,12 +316,13 @@ __global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, i
         W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
         W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
         W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
+        W.sup = q; q += n; W.kperm = q; q += n; W.nka = q; q += n; W.kb0 = q; q += n;
     }
     stedc_team(n, E + (size_t)b * n, e, W, red);
     if (threadIdx.x == 0) info[b] = 0;
 }
 
-size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 16 * sizeof(int)) + 16; }
+size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 20 * sizeof(int)) + 16; }
 
 // T factors of every reflector block: grid (blocks, instances).
 __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *work)

@@ -157,6 +157,11 @@ struct DcWork {
     int *rootpos;              // kept root j -> output column
     int *blo, *bhi;            // block [lo, hi) of every position at the current level
     int *plo, *phi;            // parent block (level L - 1; whole range at L = 0)
+    int *sup;                  // rows a column may be nonzero in: 1 first child, 2 second, 3 both
+    int *kperm;                // per block: product index k -> kept index, kept columns
+                               // supported on the first child only, on both, on the second only
+    int *nka, *kb0;            // per block at lo: rows of the first child sum k < nka,
+                               // rows of the second k >= kb0
 };
 
 MSP_HD int dc_depth(int n)
@@ -276,10 +281,13 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
 // Qn[lo+r][rootpos[lo+j]] = sum_i Q[lo+r][kcol[lo+i]] S[lo+i][lo+j] for every
 // merge block of level L (the kept columns are read through kcol in place;
 // Q and Qn hold eigenvector columns contiguously, element (row, col) at
-// col * n + row).  Large blocks (>= 48) run on FP64 matrix cores as
-// Qn^T = S^T Q_K^T, each wave computing 32 x 32 output tiles from operands
-// read in place, every operand and result access along a contiguous 16-run;
-// small ones are direct dots.
+// col * n + row).  A kept column is nonzero only on its support, so a row
+// of the first child sums the product indices k < nka and a row of the
+// second k >= kb0 (kperm orders them); without rotations that halves the
+// work.  Large blocks (>= 48) run on FP64 matrix cores as Qn^T = S^T Q_K^T,
+// each wave computing 32 x 32 output tiles from operands read in place,
+// every operand and result access along a contiguous 16-run; small ones are
+// direct dots.
 MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double *Qn, double *red)
 {
     // small blocks: one output per iteration, (column, row in its block) items
@@ -295,9 +303,13 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
             int K = W.nkept[lo];
             int j = col - lo;
             if (j < K) {
+                const bool first = row < lo + (hi - lo) / 2;
+                const int kb = first ? 0 : W.kb0[lo], ke = first ? W.nka[lo] : K;
                 double acc = 0.0;
-                for (int i = 0; i < K; ++i)
+                for (int k = kb; k < ke; ++k) {
+                    const int i = W.kperm[lo + k];
                     acc += Q[(size_t)W.kcol[lo + i] * n + row] * W.S[(size_t)(lo + i) * n + lo + j];
+                }
                 Qn[(size_t)W.rootpos[lo + j] * n + row] = acc;
             }
         }
@@ -310,13 +322,18 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
             const int K = W.nkept[blo];
 #ifdef MSP_HOST_BUILD
             (void)red;
-            for (int r = 0; r < sz; ++r)
+            for (int r = 0; r < sz; ++r) {
+                const bool first = r < sz / 2;
+                const int kb = first ? 0 : W.kb0[blo], ke = first ? W.nka[blo] : K;
                 for (int j = 0; j < K; ++j) {
                     double acc = 0.0;
-                    for (int i = 0; i < K; ++i)
+                    for (int k = kb; k < ke; ++k) {
+                        const int i = W.kperm[blo + k];
                         acc += Q[(size_t)W.kcol[blo + i] * n + blo + r] * W.S[(size_t)(blo + i) * n + blo + j];
+                    }
                     Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc;
                 }
+            }
 #else
             // 32 x 32 tiles of Qn^T = S^T Q_K^T (2 x 2 MFMA tiles of 16 x 16)
             // per wave task: rows j = kept roots, columns r = block rows.
@@ -326,17 +343,26 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
             const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
             const int li = lane & 15, lq = lane >> 4;
             const int tj = (K + 31) / 32, tr = (sz + 31) / 32;
+            const int midl = sz / 2, nka = W.nka[blo], kb0 = W.kb0[blo];
             for (int task = wave; task < tj * tr; task += nwaves) {
                 const int jt = (task / tr) * 32, rt = (task % tr) * 32;
+                // product range of this row tile: one child's rows sum their
+                // supported columns; a tile across the middle sums all with a
+                // support test per element
+                const int rend = (rt + 32 < sz) ? rt + 32 : sz;
+                const bool allA = rend <= midl, allB = rt >= midl;
+                const int kb = allB ? kb0 : 0, ke = allA ? nka : K;
                 msp_d4 acc[2][2];
                 for (int x = 0; x < 2; ++x)
                     for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
-                for (int k0 = 0; k0 < K; k0 += 4) {
+                for (int k0 = kb; k0 < ke; k0 += 4) {
                     const int kk = k0 + lq;
-                    const bool kok = kk < K;
-                    const int kc = kok ? kk : K - 1;
-                    const double *Qk = Q + (size_t)W.kcol[blo + kc] * n + blo;
-                    const double *Sk = W.S + (size_t)(blo + kc) * n + blo;
+                    const bool kok = kk < ke;
+                    const int ki = W.kperm[blo + (kok ? kk : ke - 1)];
+                    const int qc = W.kcol[blo + ki];
+                    const int sp = (allA || allB) ? 3 : W.sup[qc];
+                    const double *Qk = Q + (size_t)qc * n + blo;
+                    const double *Sk = W.S + (size_t)(blo + ki) * n + blo;
                     double a[2], bv[2];
 #pragma unroll
                     for (int x = 0; x < 2; ++x) {
@@ -347,7 +373,8 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
                         const int r = rt + 16 * x + li;
                         const int rc = (r < sz) ? r : sz - 1;
                         const double vb = Qk[rc];
-                        bv[x] = (r < sz && kok) ? vb : 0.0;
+                        const bool on = sp & ((r < midl) ? 1 : 2);
+                        bv[x] = (r < sz && kok && on) ? vb : 0.0;
                     }
 #pragma unroll
                     for (int x = 0; x < 2; ++x)
@@ -501,15 +528,47 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 }
                 W.nkept[lo] = K;
                 W.nrot[lo] = nr;
+                // supports: each column on its child's rows, a rotated one on
+                // both (its other half is zeroed before the rotations)
+                for (int c = lo; c < hi; ++c) W.sup[c] = (c < mid) ? 1 : 2;
+                for (int r = 0; r < nr; ++r) {
+                    W.sup[W.rp[lo + r]] = 3;
+                    W.sup[W.rn[lo + r]] = 3;
+                }
+                // product order of the kept columns: first child only, both,
+                // second child only, so that each child's rows sum a range
+                int c1 = 0, c3 = 0;
+                for (int i = 0; i < K; ++i) {
+                    const int sp = W.sup[W.kcol[lo + i]];
+                    c1 += (sp == 1);
+                    c3 += (sp == 3);
+                }
+                int p1 = 0, p3 = c1, p2 = c1 + c3;
+                for (int i = 0; i < K; ++i) {
+                    const int sp = W.sup[W.kcol[lo + i]];
+                    const int at = (sp == 1) ? p1++ : (sp == 3) ? p3++ : p2++;
+                    W.kperm[lo + at] = i;
+                }
+                W.nka[lo] = c1 + c3;
+                W.kb0[lo] = c1;
             }
         }
         MSP_SYNC();
         // --- apply the deflation rotations to Q rows (in scan order) ---
+        // A column holds valid entries on its child's rows only; a rotated
+        // one first gets zeros on the other child's rows.
         MSP_PFOR(row, n)
         {
             const int lo = W.blo[row], hi = W.bhi[row];
             if (hi - lo >= 2) {
                 int nr = W.nrot[lo];
+                const int mid = lo + (hi - lo) / 2;
+                const bool first = row < mid;
+                for (int r = 0; r < nr; ++r) {
+                    const int a = W.rp[lo + r], b = W.rn[lo + r];
+                    if ((a < mid) != first) Q[(size_t)a * n + row] = 0.0;
+                    if ((b < mid) != first) Q[(size_t)b * n + row] = 0.0;
+                }
                 for (int r = 0; r < nr; ++r) {
                     int a = W.rp[lo + r], b = W.rn[lo + r];
                     double c = W.rc[lo + r], s = W.rs[lo + r];
@@ -620,28 +679,29 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         }
         MSP_SYNC();
         // --- new Q columns ---
-        // Q is block diagonal.  Each pass runs over (column, row of the
-        // column's parent block) items: rows inside the column's block get a
-        // copy (deflated columns, untouched blocks) or, for kept roots, the
-        // product Q[:, kcol] S (multiplied below, Q read in place); rows of
-        // the parent block outside it are set to zero, because the next
-        // level's rotations and products read them.
+        // Q is block diagonal, and only the rows of a column's block are
+        // written: deflated columns (on their support, zero on the other
+        // child's rows) and untouched blocks are copied here; kept roots are
+        // the products Q[:, kcol] S below.  Rows of the parent block outside
+        // the column's block keep stale values, which the next level never
+        // reads (its supports).
         {
-            const int pmax = (L > 0) ? dc_maxblock(n, L - 1) : n;
+            const int smax = dc_maxblock(n, L);
             int sh = 0;
-            while ((1 << sh) < pmax) ++sh;
+            while ((1 << sh) < smax) ++sh;
             MSP_PFOR(idx, n << sh)
             {
                 const int col = idx >> sh;
                 const int lo = W.blo[col], hi = W.bhi[col];
-                const int plo = W.plo[col], phi = W.phi[col];
-                const int row = plo + (idx & ((1 << sh) - 1));
-                if (row < phi) {
-                    if (row < lo || row >= hi) {
-                        Qn[(size_t)col * n + row] = 0.0;
-                    } else if (hi - lo >= 2) {
+                const int row = lo + (idx & ((1 << sh) - 1));
+                if (row < hi) {
+                    if (hi - lo >= 2) {
                         const int src = W.outsrc[col];
-                        if (src < 0) Qn[(size_t)col * n + row] = Q[(size_t)W.perm[~src] * n + row];
+                        if (src < 0) {
+                            const int c = W.perm[~src];
+                            const int sd = (row < lo + (hi - lo) / 2) ? 1 : 2;
+                            Qn[(size_t)col * n + row] = (W.sup[c] & sd) ? Q[(size_t)c * n + row] : 0.0;
+                        }
                     } else {
                         Qn[(size_t)col * n + row] = Q[(size_t)col * n + row];
                     }
@@ -725,10 +785,10 @@ MSP_HD void backtransform_team(int n, const cdbl *A, const cdbl *tau, const doub
 MSP_HD int bt_nblocks(int n) { return (n - 1 + MSP_BT_NB - 1) / MSP_BT_NB; }
 
 // Doubles of per-matrix global workspace used by the divide-and-conquer path:
-// e (n), tau (2n), DcWork doubles (3n^2 + 12n) and ints (11n, padded).
+// e (n), tau (2n), DcWork doubles (4n^2 + 12n) and ints (20n, padded).
 MSP_HD size_t dc_work_doubles(int n)
 {
-    return (size_t)4 * n * n + 15 * (size_t)n + 9 * (size_t)n + 8 +
+    return (size_t)4 * n * n + 15 * (size_t)n + 11 * (size_t)n + 8 +
            2 * (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB + 4 * (size_t)n * MSP_TD_NB;
 }
 
@@ -751,13 +811,14 @@ MSP_HD DcWork dc_carve(int n, double *base, double **e, cdbl **tau)
     W.nkept = q; q += n; W.rp = q; q += n; W.rn = q; q += n; W.nrot = q; q += n;
     W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
     W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
+    W.sup = q; q += n; W.kperm = q; q += n; W.nka = q; q += n; W.kb0 = q; q += n;
     return W;
 }
 
 // T factors of the blocked back-transformation (after the DcWork arrays).
 MSP_HD cdbl *dc_tblocks(int n, double *base)
 {
-    return (cdbl *)(base + (size_t)4 * n * n + 15 * (size_t)n + 9 * (size_t)n + 8);
+    return (cdbl *)(base + (size_t)4 * n * n + 15 * (size_t)n + 11 * (size_t)n + 8);
 }
 
 // Panel buffers Vp, Wp (n x MSP_TD_NB complex each) of the blocked reduction.
