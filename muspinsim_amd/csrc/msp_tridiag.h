@@ -446,6 +446,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 if (i < mid) zi = Q[(size_t)i * n + mid - 1];
                 else zi = (ec < 0.0 ? -1.0 : 1.0) * Q[(size_t)i * n + mid];
                 W.zv[i] = zi * 0.7071067811865476;
+                W.sup[i] = (i < mid) ? 1 : 2;      // support: the column's child
                 // rank within the merged (two sorted halves) order
                 double di = W.dv[i];
                 int r;
@@ -528,29 +529,12 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 }
                 W.nkept[lo] = K;
                 W.nrot[lo] = nr;
-                // supports: each column on its child's rows, a rotated one on
-                // both (its other half is zeroed before the rotations)
-                for (int c = lo; c < hi; ++c) W.sup[c] = (c < mid) ? 1 : 2;
+                // a rotated column is supported on both children (its other
+                // half is zeroed before the rotations)
                 for (int r = 0; r < nr; ++r) {
                     W.sup[W.rp[lo + r]] = 3;
                     W.sup[W.rn[lo + r]] = 3;
                 }
-                // product order of the kept columns: first child only, both,
-                // second child only, so that each child's rows sum a range
-                int c1 = 0, c3 = 0;
-                for (int i = 0; i < K; ++i) {
-                    const int sp = W.sup[W.kcol[lo + i]];
-                    c1 += (sp == 1);
-                    c3 += (sp == 3);
-                }
-                int p1 = 0, p3 = c1, p2 = c1 + c3;
-                for (int i = 0; i < K; ++i) {
-                    const int sp = W.sup[W.kcol[lo + i]];
-                    const int at = (sp == 1) ? p1++ : (sp == 3) ? p3++ : p2++;
-                    W.kperm[lo + at] = i;
-                }
-                W.nka[lo] = c1 + c3;
-                W.kb0[lo] = c1;
             }
         }
         MSP_SYNC();
@@ -592,6 +576,23 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     double t = secular_root(K, W.dl + lo, W.zl + lo, rho, j, org);
                     W.korg[s] = org;
                     W.ktau[s] = t;
+                    // product order of the kept columns: first child only,
+                    // both, second child only, so that each child's rows sum
+                    // a range of it
+                    const int cls = (W.sup[W.kcol[s]] == 1) ? 0 : (W.sup[W.kcol[s]] == 3) ? 1 : 2;
+                    int pos = 0, c1 = 0, c3 = 0;
+                    for (int i = 0; i < K; ++i) {
+                        const int sp = W.sup[W.kcol[lo + i]];
+                        const int ci = (sp == 1) ? 0 : (sp == 3) ? 1 : 2;
+                        pos += (ci < cls) || (ci == cls && i < j);
+                        c1 += (ci == 0);
+                        c3 += (ci == 1);
+                    }
+                    W.kperm[lo + pos] = j;
+                    if (j == 0) {
+                        W.nka[lo] = c1 + c3;
+                        W.kb0[lo] = c1;
+                    }
                 }
             }
         }
