@@ -9,11 +9,13 @@
 // The accumulator of a tile is also the B operand of W = V^H U, because the
 // accumulator slot r of lane l holds U[4 r + l/16][l%16] -- the rows of the
 // k-step r of that tile.  Per reflector block:
-//   W  = V^H U      16 x 16, partial per wave over its tiles, summed in LDS
-//   W2 = T W        16 x 16, VALU, broadcast through LDS
-//   U -= V W2       in the accumulators of every tile below the block
-// U never leaves registers, so LDS holds only the W partials and W2, and
-// several workgroups share a CU.
+//   W  = V^H U      16 x 16, partial per wave over its tiles, into LDS
+//   U -= (V T) W    in the accumulators of every tile below the block; each
+//                   wave sums the W partials itself, V T comes from the WY
+//                   factor kernel
+// One barrier per block (the W partials alternate between two LDS buffers).
+// U never leaves registers, so LDS holds only the W partials, and several
+// workgroups share a CU.
 #include <hip/hip_runtime.h>
 
 #include "msp_kernels.h"
@@ -39,11 +41,10 @@ __device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j
 }
 
 template <int RT, int NW>
-__global__ void __launch_bounds__(64 * NW) k_backtransform_reg(int n, const double *H, double *work,
+__global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backtransform_reg(int n, const double *H, double *work,
                                                                double *U)
 {
-    __shared__ double Wp[NW][2][256];         // per-wave partial W (re, im)
-    __shared__ double W2[2][256];             // T W (re, im), [row j][col c]
+    __shared__ double Wp[2][NW][2][256];      // per-wave partial W (re, im), by block parity
     const int b = blockIdx.y;
     const int c0 = blockIdx.x * CW;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(64 * NW) k_backtransform_reg(int n, const doub
     const cdbl *A = (const cdbl *)H + (size_t)b * n * n;
     double *base = work + (size_t)b * dc_work_doubles(n);
     const double *Z = dc_result_ptr(n, base);
-    const cdbl *Tall = dc_tblocks(n, base);
+    const cdbl *VTall = dc_vt(n, base);
 
     msp_d4 ur[RT], ui[RT];
 #pragma unroll
@@ -89,47 +90,40 @@ __global__ void __launch_bounds__(64 * NW) k_backtransform_reg(int n, const doub
             // in flight at a time
             if constexpr (NW > 4) __builtin_amdgcn_sched_barrier(0);
         }
+        const int par = blk & 1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            Wp[wave][0][(4 * q + lq) * 16 + li] = wr[q];
-            Wp[wave][1][(4 * q + lq) * 16 + li] = wi[q];
+            Wp[par][wave][0][(4 * q + lq) * 16 + li] = wr[q];
+            Wp[par][wave][1][(4 * q + lq) * 16 + li] = wi[q];
         }
         __syncthreads();
-        // ---- W2 = T W, one element per thread of the first 256
-        if (tid < 256) {
-            const int j = tid / 16, c = tid % 16;
-            const cdbl *T = Tall + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
-            double sr = 0.0, si = 0.0;
-            for (int l = j; l < nbk; ++l) {
-                const int o = l * 16 + c;
-                double wre = 0.0, wim = 0.0;
-#pragma unroll
-                for (int w2 = 0; w2 < NW; ++w2) {
-                    wre += Wp[w2][0][o];
-                    wim += Wp[w2][1][o];
-                }
-                const cdbl tt = T[j * MSP_BT_NB + l];
-                sr += tt.re * wre - tt.im * wim;
-                si += tt.re * wim + tt.im * wre;
-            }
-            W2[0][j * 16 + c] = (j < nbk) ? sr : 0.0;
-            W2[1][j * 16 + c] = (j < nbk) ? si : 0.0;
-        }
-        __syncthreads();
-        // ---- U -= V W2 in the accumulators of the tiles below the block
+        // ---- U -= (V T) W in the accumulators of the tiles below the block
         double br[4], bi[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            br[s] = W2[0][(4 * s + lq) * 16 + li];
-            bi[s] = W2[1][(4 * s + lq) * 16 + li];
+            const int o = (4 * s + lq) * 16 + li;
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < NW; ++w2) {
+                sr += Wp[par][w2][0][o];
+                si += Wp[par][w2][1][o];
+            }
+            br[s] = sr;
+            bi[s] = si;
         }
+        const cdbl *VT = VTall + (size_t)blk * MSP_BT_NB * n;
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
             const int t = wave + NW * i;
             if (t < ntiles && 16 * t + 15 >= k0 + 1) {
+                const int r = 16 * t + li;
+                const int rc = (r < n) ? r : n - 1;
                 cdbl v[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + li, 4 * s + lq);
+                for (int s = 0; s < 4; ++s) {
+                    const cdbl x = VT[(size_t)(4 * s + lq) * n + rc];
+                    v[s] = (r < n) ? x : cmk(0.0, 0.0);
+                }
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
                     mfma_cmac(-v[s].re, -v[s].im, br[s], bi[s], ur[i], ui[i]);

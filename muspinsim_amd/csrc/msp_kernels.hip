@@ -324,19 +324,89 @@ __global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, i
 
 size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 20 * sizeof(int)) + 16; }
 
-// T factors of every reflector block: grid (blocks, instances).
+// T factors of every reflector block, and V T for the MFMA back-transformation:
+// grid (blocks, instances).  With STAGE the block's reflectors are first
+// copied to LDS (n x 16, rows padded to 17), so that the Gram matrix and V T
+// read them from there instead of from strided global rows.
+template <bool STAGE>
 __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *work)
 {
     __shared__ cdbl G[MSP_BT_NB * MSP_BT_NB];
     __shared__ cdbl T[MSP_BT_NB * MSP_BT_NB];
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int LDV = MSP_BT_NB + 1;
     const int b = blockIdx.y, blk = blockIdx.x;
     double *base = work + (size_t)b * dc_work_doubles(n);
     const cdbl *tau = dc_tau_ptr(n, base);
     const int k0 = blk * MSP_BT_NB;
     const int nbk = (n - 1 - k0 < MSP_BT_NB) ? n - 1 - k0 : MSP_BT_NB;
-    larft_block_team(n, (const cdbl *)H + (size_t)b * n * n, tau, k0, nbk, MSP_BT_NB, T, G);
+    const cdbl *A = (const cdbl *)H + (size_t)b * n * n;
+    cdbl *Vs = (cdbl *)smem;
+    if constexpr (STAGE) {
+        MSP_PFOR(idx, n * MSP_BT_NB)
+        {
+            const int r = idx / MSP_BT_NB, l = idx % MSP_BT_NB;
+            Vs[r * LDV + l] = (l < nbk) ? vblock(n, A, k0, r, l) : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        MSP_PFOR(idx, nbk * nbk)
+        {
+            const int i = idx / nbk, j = idx % nbk;
+            cdbl acc = cmk(0.0, 0.0);
+            if (i < j)
+                for (int r = k0 + j + 1; r < n; ++r) cfmac(acc, Vs[r * LDV + i], Vs[r * LDV + j]);
+            G[i * MSP_BT_NB + j] = acc;
+        }
+        MSP_PFOR(idx, MSP_BT_NB * MSP_BT_NB) T[idx] = cmk(0.0, 0.0);
+        __syncthreads();
+        for (int j = 0; j < nbk; ++j) {
+            const cdbl tj = tau[k0 + j];
+            MSP_PFOR(i, j + 1)
+            {
+                if (i == j) {
+                    T[j * MSP_BT_NB + j] = tj;
+                } else {
+                    cdbl acc = cmk(0.0, 0.0);
+                    for (int l = i; l < j; ++l) cfma(acc, T[i * MSP_BT_NB + l], G[l * MSP_BT_NB + j]);
+                    T[i * MSP_BT_NB + j] = cmul(cmk(-tj.re, -tj.im), acc);
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        larft_block_team(n, A, tau, k0, nbk, MSP_BT_NB, T, G);
+    }
     cdbl *Tout = dc_tblocks(n, base) + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
     MSP_PFOR(idx, MSP_BT_NB * MSP_BT_NB) Tout[idx] = T[idx];
+    // V T (T upper triangular), so that the MFMA back-transformation applies
+    // U -= (V T)(V^H U) without a T W step
+    cdbl *VT = dc_vt(n, base) + (size_t)blk * MSP_BT_NB * n;
+    MSP_PFOR(idx, MSP_BT_NB * n)
+    {
+        const int c = idx / n, r = idx % n;
+        cdbl acc = cmk(0.0, 0.0);
+        const int lmax = (c < nbk) ? c : nbk - 1;
+        for (int l = 0; l <= lmax; ++l)
+            cfma(acc, STAGE ? Vs[r * LDV + l] : vblock(n, A, k0, r, l), T[l * MSP_BT_NB + c]);
+        VT[idx] = acc;
+    }
+}
+
+// LDS-staged WY factors up to this order (n x 17 complex, 52 KB at n = 192)
+constexpr int kLarftStageMax = 256;
+
+hipError_t launch_larft(int n, int batch, const double *H, double *work, hipStream_t s)
+{
+    if (n <= kLarftStageMax) {
+        const size_t lds = sizeof(cdbl) * (size_t)n * (MSP_BT_NB + 1);
+        hipError_t err = hipFuncSetAttribute((const void *)k_larft<true>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(k_larft<true>, dim3(bt_nblocks(n), batch), dim3(256), lds, s, n, H, work);
+    } else {
+        hipLaunchKernelGGL(k_larft<false>, dim3(bt_nblocks(n), batch), dim3(256), 0, s, n, H, work);
+    }
+    return hipGetLastError();
 }
 
 // U = Q Z on column chunks of width cw: grid (chunks, instances).
@@ -818,10 +888,9 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
         if ((err = hipStreamWaitEvent(s2, ev_td, 0)) != hipSuccess) return err;
         {
             ProfScope ps_("k_larft", s2);
-            hipLaunchKernelGGL(k_larft, dim3(bt_nblocks(n), batch), dim3(256), 0, s2, n,
-                               (const double *)H, work);
+            err = launch_larft(n, batch, (const double *)H, work, s2);
         }
-        if ((err = hipGetLastError()) != hipSuccess) return err;
+        if (err != hipSuccess) return err;
         if ((err = hipEventRecord(ev_ft, s2)) != hipSuccess) return err;
     }
     {
@@ -841,8 +910,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
         if (err != hipSuccess) return err;
     } else if (bt_nblocks(n) > 0) {
         ProfScope ps_("k_larft", s);
-        hipLaunchKernelGGL(k_larft, dim3(bt_nblocks(n), batch), dim3(256), 0, s, n,
-                           (const double *)H, work);
+        if ((err = launch_larft(n, batch, (const double *)H, work, s)) != hipSuccess) return err;
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (n <= 1024) {

@@ -790,7 +790,8 @@ MSP_HD int bt_nblocks(int n) { return (n - 1 + MSP_BT_NB - 1) / MSP_BT_NB; }
 MSP_HD size_t dc_work_doubles(int n)
 {
     return (size_t)4 * n * n + 15 * (size_t)n + 11 * (size_t)n + 8 +
-           2 * (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB + 4 * (size_t)n * MSP_TD_NB;
+           2 * (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB + 4 * (size_t)n * MSP_TD_NB +
+           2 * (size_t)bt_nblocks(n) * MSP_BT_NB * n;
 }
 
 // Carve a DcWork (and e, tau) out of one contiguous per-matrix block.
@@ -826,6 +827,13 @@ MSP_HD cdbl *dc_tblocks(int n, double *base)
 MSP_HD cdbl *dc_panels(int n, double *base)
 {
     return dc_tblocks(n, base) + (size_t)(bt_nblocks(n) + 1) * MSP_BT_NB * MSP_BT_NB;
+}
+
+// (V T) of every reflector block, column by column: element (r, c) of block
+// blk at [(blk * MSP_BT_NB + c) * n + r].
+MSP_HD cdbl *dc_vt(int n, double *base)
+{
+    return dc_panels(n, base) + (size_t)2 * n * MSP_TD_NB;
 }
 
 // Direct offsets into the per-matrix workspace (same layout as dc_carve), for
