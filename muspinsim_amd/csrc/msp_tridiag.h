@@ -213,13 +213,23 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
     const double eps = 2.220446049250313e-16;
     int org;
     double lo, hi;
-    if (j < K - 1) {
+    // the secular function at the midpoint of an interior gap, split into the
+    // poles left (psi) and right (phi) of the root, picks the origin pole and
+    // is the first iterate of the model steps below
+    double m_psi = 0.0, m_dpsi = 0.0, m_phi = 0.0, m_dphi = 0.0, m_tau = 0.0;
+    const bool interior = j < K - 1;
+    if (interior) {
         double gap = dl[j + 1] - dl[j];
         double mid = 0.5 * gap;
-        double f = 1.0;
-        for (int i = 0; i < K; ++i) f += rho * zl[i] * zl[i] / ((dl[i] - dl[j]) - mid);
-        if (f > 0.0) { org = j; lo = 0.0; hi = mid; }
-        else { org = j + 1; lo = -mid; hi = 0.0; }
+        for (int i = 0; i < K; ++i) {
+            const double r = msp_recip((dl[i] - dl[j]) - mid);
+            const double t = rho * zl[i] * zl[i] * r;
+            if (i <= j) { m_psi += t; m_dpsi += t * r; }
+            else { m_phi += t; m_dphi += t * r; }
+        }
+        double f = 1.0 + m_psi + m_phi;
+        if (f > 0.0) { org = j; lo = 0.0; hi = mid; m_tau = mid; }
+        else { org = j + 1; lo = -mid; hi = 0.0; m_tau = -mid; }
     } else {
         double z2 = 0.0;
         for (int i = 0; i < K; ++i) z2 += zl[i] * zl[i];
@@ -233,16 +243,24 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
     // interior model uses the two poles j and j+1 around the root
     const int pl = (j < K - 1) ? j : K - 1;
     const int pr = (j < K - 1) ? j + 1 : -1;
-    double tau = 0.5 * (lo + hi);
+    double tau = interior ? m_tau : 0.5 * (lo + hi);
     for (int it = 0; it < 80; ++it) {
         // psi / phi: poles left / right of the root; with r = 1/del,
-        // term = rho z^2 r and its derivative rho z^2 r^2
+        // term = rho z^2 r and its derivative rho z^2 r^2 (the first interior
+        // iterate reuses the midpoint evaluation)
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-        for (int i = 0; i < K; ++i) {
-            const double r = msp_recip((dl[i] - dorg) - tau);
-            const double t = rho * zl[i] * zl[i] * r;
-            if (i <= pl) { psi += t; dpsi += t * r; }
-            else { phi += t; dphi += t * r; }
+        if (it == 0 && interior) {
+            psi = m_psi;
+            dpsi = m_dpsi;
+            phi = m_phi;
+            dphi = m_dphi;
+        } else {
+            for (int i = 0; i < K; ++i) {
+                const double r = msp_recip((dl[i] - dorg) - tau);
+                const double t = rho * zl[i] * zl[i] * r;
+                if (i <= pl) { psi += t; dpsi += t * r; }
+                else { phi += t; dphi += t * r; }
+            }
         }
         double w = 1.0 + psi + phi;
         // |w| within its rounding error: tau is the root to working precision
@@ -612,7 +630,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                         int org = W.korg[lo + j];
                         double del = (di - dl[org]) - W.ktau[lo + j];  // d_i - lambda_j
                         if (j == i) wv *= del;
-                        else wv *= del / (di - dl[j]);
+                        else wv *= del * msp_recip(di - dl[j]);
                     }
                     double zz = sqrt(fabs(wv));
                     W.zh[s] = (W.zl[s] < 0.0) ? -zz : zz;
@@ -631,15 +649,17 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     const double *dl = W.dl + lo;
                     int org = W.korg[s];
                     double tj = W.ktau[s];
+                    // the norm first, then each entry once (recomputed, not
+                    // read back)
                     double nrm = 0.0;
                     for (int i = 0; i < K; ++i) {
-                        double del = (dl[i] - dl[org]) - tj;
-                        double u = W.zh[lo + i] / del;
-                        W.S[(size_t)(lo + i) * n + (lo + j)] = u;
+                        const double u = W.zh[lo + i] * msp_recip((dl[i] - dl[org]) - tj);
                         nrm += u * u;
                     }
                     nrm = 1.0 / sqrt(nrm);
-                    for (int i = 0; i < K; ++i) W.S[(size_t)(lo + i) * n + (lo + j)] *= nrm;
+                    for (int i = 0; i < K; ++i)
+                        W.S[(size_t)(lo + i) * n + (lo + j)] =
+                            W.zh[lo + i] * nrm * msp_recip((dl[i] - dl[org]) - tj);
                 }
             }
         }
