@@ -357,22 +357,29 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
                 for (int r = k0 + j + 1; r < n; ++r) cfmac(acc, Vs[r * LDV + i], Vs[r * LDV + j]);
             G[i * MSP_BT_NB + j] = acc;
         }
-        MSP_PFOR(idx, MSP_BT_NB * MSP_BT_NB) T[idx] = cmk(0.0, 0.0);
         __syncthreads();
-        for (int j = 0; j < nbk; ++j) {
-            const cdbl tj = tau[k0 + j];
-            MSP_PFOR(i, j + 1)
-            {
-                if (i == j) {
-                    T[j * MSP_BT_NB + j] = tj;
-                } else {
-                    cdbl acc = cmk(0.0, 0.0);
-                    for (int l = i; l < j; ++l) cfma(acc, T[i * MSP_BT_NB + l], G[l * MSP_BT_NB + j]);
-                    T[i * MSP_BT_NB + j] = cmul(cmk(-tj.re, -tj.im), acc);
+        // T = (diag(1/tau) + strictly upper part of V^H V)^{-1}, by back
+        // substitution one column per thread, multiplied through by tau_i so
+        // that tau_i = 0 (no reflection) gives a zero row:
+        //   T(j, j) = tau_j,  T(i, j) = -tau_i sum_{k = i+1..j} G(i, k) T(k, j)
+        if (threadIdx.x < MSP_BT_NB) {
+            const int j = threadIdx.x;
+            for (int i = MSP_BT_NB - 1; i >= 0; --i) {
+                cdbl t = cmk(0.0, 0.0);
+                if (j < nbk && i <= j) {
+                    if (i == j) {
+                        t = tau[k0 + j];
+                    } else {
+                        cdbl acc = cmk(0.0, 0.0);
+                        for (int k = i + 1; k <= j; ++k) cfma(acc, G[i * MSP_BT_NB + k], T[k * MSP_BT_NB + j]);
+                        const cdbl ti = tau[k0 + i];
+                        t = cmul(cmk(-ti.re, -ti.im), acc);
+                    }
                 }
+                T[i * MSP_BT_NB + j] = t;
             }
-            __syncthreads();
         }
+        __syncthreads();
     } else {
         larft_block_team(n, A, tau, k0, nbk, MSP_BT_NB, T, G);
     }
@@ -381,14 +388,33 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
     // V T (T upper triangular), so that the MFMA back-transformation applies
     // U -= (V T)(V^H U) without a T W step
     cdbl *VT = dc_vt(n, base) + (size_t)blk * MSP_BT_NB * n;
-    MSP_PFOR(idx, MSP_BT_NB * n)
-    {
-        const int c = idx / n, r = idx % n;
-        cdbl acc = cmk(0.0, 0.0);
-        const int lmax = (c < nbk) ? c : nbk - 1;
-        for (int l = 0; l <= lmax; ++l)
-            cfma(acc, STAGE ? Vs[r * LDV + l] : vblock(n, A, k0, r, l), T[l * MSP_BT_NB + c]);
-        VT[idx] = acc;
+    if constexpr (STAGE) {
+        // one row of V per thread, held in registers; T entries are LDS
+        // broadcasts; the VT columns are written along r (coalesced)
+        MSP_PFOR(r, n)
+        {
+            cdbl v[MSP_BT_NB];
+#pragma unroll
+            for (int l = 0; l < MSP_BT_NB; ++l) v[l] = Vs[r * LDV + l];
+            // (T is upper triangular: the terms l > c add zeros, and the
+            // full l loop keeps v in registers with static indices)
+#pragma unroll 1
+            for (int c = 0; c < MSP_BT_NB; ++c) {
+                cdbl acc = cmk(0.0, 0.0);
+#pragma unroll
+                for (int l = 0; l < MSP_BT_NB; ++l) cfma(acc, v[l], T[l * MSP_BT_NB + c]);
+                VT[(size_t)c * n + r] = acc;
+            }
+        }
+    } else {
+        MSP_PFOR(idx, MSP_BT_NB * n)
+        {
+            const int c = idx / n, r = idx % n;
+            cdbl acc = cmk(0.0, 0.0);
+            const int lmax = (c < nbk) ? c : nbk - 1;
+            for (int l = 0; l <= lmax; ++l) cfma(acc, vblock(n, A, k0, r, l), T[l * MSP_BT_NB + c]);
+            VT[idx] = acc;
+        }
     }
 }
 
