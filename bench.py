@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
     ap.add_argument("--chunk-gib", type=float, default=0.0,
                     help="device memory budget of one chunk in GiB (0 = engine default)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the plan from a captured HIP graph (SpinEngine.execute_graph)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy a device buffer of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -159,26 +161,36 @@ def main():
         kw["chunk_bytes"] = int(args.chunk_gib * (1 << 30))
     eng = SpinEngine("cuda:0", **kw)
     plan = eng.prepare(systems, batch)
+    run = eng.execute_graph if args.graph else eng.execute
     for _ in range(args.warmup):
-        eng.execute(plan)
+        run(plan)
     torch.cuda.synchronize()
     N.profile_report()                      # clear
-    N.load_library().msp_profile_enable(1)
+    # per-kernel HIP-event times come from the timed steps themselves; graph
+    # replays record no events, so with --graph one extra eager step
+    # (untimed) supplies them
+    N.load_library().msp_profile_enable(0 if args.graph else 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.execute(plan)
+        run(plan)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    N.load_library().msp_profile_enable(0)
-    prof = N.profile_report()
     results = plan.results.cpu().numpy()
     rows = plan.rows.cpu().numpy()
+    prof_steps = args.steps
+    if args.graph:
+        N.load_library().msp_profile_enable(1)
+        eng.execute(plan)
+        torch.cuda.synchronize()
+        prof_steps = 1
+    N.load_library().msp_profile_enable(0)
+    prof = N.profile_report()
 
     # untimed pass without event bracketing, to report the overhead-free rate
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    eng.execute(plan)
+    run(plan)
     torch.cuda.synchronize()
     dt_clean = time.perf_counter() - t1
 
@@ -191,7 +203,7 @@ def main():
     work = sum(algorithmic(kname, int(d), width)[1] * c
                for d, c in zip(*np.unique(dims, return_counts=True)))
     per_inst = work / ninst
-    inst_per_launch = ninst * args.steps / launches
+    inst_per_launch = ninst * prof_steps / launches
     avg_s = tot_ms / launches / 1e3
     if unit == "flop":
         achieved = per_inst * inst_per_launch / avg_s / 1e12
@@ -224,7 +236,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded couplings and orientations)",
-        "config": {"workload": wname, "hilbert_dim": n if dims.min() == n else
+        "config": {"workload": wname, "launch": "hip-graph replay" if args.graph else "stream", "hilbert_dim": n if dims.min() == n else
                    f"{int(dims.min())}-{n}", "instances": ninst,
                    "precision": "fp64-plain",
                    "eigensolver": "hessenberg+complex-schur (Liouvillian)"
@@ -233,7 +245,7 @@ def main():
                    else "jacobi-lds" if n <= 64 else "jacobi-lds + householder+divide-and-conquer"},
         "max_abs_dP_vs_fp64_oracle": maxdp,
         "value_without_event_timing": ninst / dt_clean,
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in sorted(prof.items())},
+        "kernels_ms_per_step": {k: v[1] / prof_steps for k, v in sorted(prof.items())},
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

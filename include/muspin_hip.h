@@ -227,6 +227,19 @@ int msp_trace_products(int n, int nparam, const double *d_G, const double *d_dH,
 int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
                    const double *d_rows, double scale, double *d_results, void *stream);
 
+/* Graph capture and replay of a sequence of msp_* calls on one stream (no
+ * reference counterpart: the reference re-runs Python per instance).  Between
+ * msp_graph_begin and msp_graph_end the calls on `stream` are recorded, not
+ * run (including the side-stream work msp_eigh forks and joins); run the same
+ * sequence once eagerly first, so that no kernel attribute is set during the
+ * capture.  msp_graph_end instantiates the graph into *graph_exec;
+ * msp_graph_launch replays it on a stream; msp_graph_destroy frees it.  The
+ * buffers the calls name must stay allocated while the graph is in use. */
+int msp_graph_begin(void *stream);
+int msp_graph_end(void *stream, void **graph_exec);
+int msp_graph_launch(void *graph_exec, void *stream);
+int msp_graph_destroy(void *graph_exec);
+
 /* Kernel timing (no reference counterpart; instrumentation for benchmarks):
  * while enabled every kernel launch is bracketed by two hipEvents on its
  * stream.  msp_profile_report waits for the recorded events, writes lines

@@ -72,6 +72,10 @@ SIGNATURES = {
     "msp_batch_sum": (_I, [_I, _I, _P, _P, _P]),
     "msp_trace_products": (_I, [_I, _I, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
+    "msp_graph_begin": (_I, [_P]),
+    "msp_graph_end": (_I, [_P, ctypes.POINTER(_P)]),
+    "msp_graph_launch": (_I, [_P, _P]),
+    "msp_graph_destroy": (_I, [_P]),
     "msp_profile_enable": (_I, [_I]),
     "msp_profile_report": (_I, [ctypes.c_char_p, _SZ]),
 }

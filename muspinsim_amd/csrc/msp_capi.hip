@@ -48,6 +48,39 @@ int msp_last_error(char *h_buf, size_t len)
     return MSP_OK;
 }
 
+int msp_graph_begin(void *stream)
+{
+    if (!stream) return fail(MSP_ERR_VALUE, "msp_graph_begin: the legacy default stream cannot be captured");
+    return hip_status(hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal),
+                      "msp_graph_begin");
+}
+
+int msp_graph_end(void *stream, void **graph_exec)
+{
+    if (!stream || !graph_exec) return fail(MSP_ERR_VALUE, "msp_graph_end: null stream or output");
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(S(stream), &g);
+    if (e != hipSuccess) return hip_status(e, "msp_graph_end");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_status(e, "msp_graph_end");
+    *graph_exec = (void *)x;
+    return MSP_OK;
+}
+
+int msp_graph_launch(void *graph_exec, void *stream)
+{
+    if (!graph_exec) return fail(MSP_ERR_VALUE, "msp_graph_launch: null graph");
+    return hip_status(hipGraphLaunch((hipGraphExec_t)graph_exec, S(stream)), "msp_graph_launch");
+}
+
+int msp_graph_destroy(void *graph_exec)
+{
+    if (!graph_exec) return MSP_OK;
+    return hip_status(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "msp_graph_destroy");
+}
+
 int msp_init(void)
 {
     int dev = 0;

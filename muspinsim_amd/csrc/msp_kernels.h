@@ -20,6 +20,11 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// Raise a kernel's dynamic LDS limit to `bytes` unless an earlier call on
+// this device already did (the attribute is not a stream operation; after a
+// first eager run, launches under stream capture make no such call).
+hipError_t msp_set_lds(const void *func, size_t bytes);
+
 size_t jacobi_lds_bytes(int n);
 size_t dc_workspace_doubles(int n);
 int dc_chunk_width(int n);

@@ -425,8 +425,7 @@ hipError_t launch_larft(int n, int batch, const double *H, double *work, hipStre
 {
     if (n <= kLarftStageMax) {
         const size_t lds = sizeof(cdbl) * (size_t)n * (MSP_BT_NB + 1);
-        hipError_t err = hipFuncSetAttribute((const void *)k_larft<true>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t err = msp_set_lds((const void *)k_larft<true>, lds);
         if (err != hipSuccess) return err;
         hipLaunchKernelGGL(k_larft<true>, dim3(bt_nblocks(n), batch), dim3(256), lds, s, n, H, work);
     } else {
@@ -856,13 +855,37 @@ hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, doub
     if (nt < 64) nt = 64;
     if (nt > 1024) nt = 1024;
     size_t lds = jacobi_lds_bytes(n);
-    hipError_t err = hipFuncSetAttribute((const void *)k_eigh_jacobi,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t err = msp_set_lds((const void *)k_eigh_jacobi, lds);
     if (err != hipSuccess) return err;
     { ProfScope ps_("k_eigh_jacobi", s);
     hipLaunchKernelGGL(k_eigh_jacobi, dim3(batch), dim3(nt), lds, s, n, H, E, U, info, max_sweeps);
     }
     return hipGetLastError();
+}
+
+hipError_t msp_set_lds(const void *func, size_t bytes)
+{
+    struct Rec {
+        const void *f;
+        int dev;
+        size_t bytes;
+    };
+    static std::vector<Rec> done;
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    std::lock_guard<std::mutex> lock(mu);
+    for (Rec &r : done)
+        if (r.f == func && r.dev == dev) {
+            if (bytes <= r.bytes) return hipSuccess;
+            err = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            if (err == hipSuccess) r.bytes = bytes;
+            return err;
+        }
+    err = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (err == hipSuccess) done.push_back({func, dev, bytes});
+    return err;
 }
 
 // Second stream of each device, for work that overlaps the divide and conquer
@@ -888,8 +911,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
         err = launch_tridiag_lp(n, batch, H, E, work, s);
     } else {
         size_t lds1 = sizeof(cdbl) * 2 * (size_t)n + 16 * sizeof(double);
-        err = hipFuncSetAttribute((const void *)k_tridiag,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+        err = msp_set_lds((const void *)k_tridiag, lds1);
         if (err != hipSuccess) return err;
         ProfScope ps_("k_tridiag", s);
         hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(256), lds1, s, n, H, E, work);
@@ -922,8 +944,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     {
         const int in_lds = (n <= 512) ? 1 : 0;
         const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
-        err = hipFuncSetAttribute((const void *)k_stedc, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds2);
+        err = msp_set_lds((const void *)k_stedc, lds2);
         if (err != hipSuccess) return err;
         ProfScope ps_("k_stedc", s);
         hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
@@ -945,8 +966,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     }
     const int cw = bt2_chunk_width(n);
     size_t lds3 = sizeof(cdbl) * ((size_t)n * (cw + 1) + 2 * MSP_BT_NB * cw);
-    err = hipFuncSetAttribute((const void *)k_backtransform2,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3);
+    err = msp_set_lds((const void *)k_backtransform2, lds3);
     if (err != hipSuccess) return err;
     { ProfScope ps_("k_backtransform", s);
     hipLaunchKernelGGL(k_backtransform2, dim3(cdiv(n, cw), batch), dim3(256), lds3, s, n,

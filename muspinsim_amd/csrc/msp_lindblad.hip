@@ -222,8 +222,7 @@ hipError_t launch_lindblad_modes(int N, int batch, double *L, const double *r, i
 {
     const int in_lds = N <= kLdsMaxN;
     const size_t lds = in_lds ? sizeof(cdbl) * (2 * (size_t)N * (N + 1) + modes_vec_elems(N)) : 0;
-    hipError_t err = hipFuncSetAttribute((const void *)k_lindblad_modes,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t err = msp_set_lds((const void *)k_lindblad_modes, lds);
     if (err != hipSuccess) return err;
     ProfScope ps_("k_lindblad_modes", s);
     hipLaunchKernelGGL(k_lindblad_modes, dim3(batch), dim3(kNT), lds, s, N, L, r, n_ops, u, lam,

@@ -421,8 +421,7 @@ template <int NB, int QMAX, int RB, int Q16, bool GP = false>
 hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
     size_t lds = lp_lds<NB, GP>(n);
-    hipError_t err = hipFuncSetAttribute((const void *)k_tridiag_lp<NB, QMAX, RB, Q16, GP>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t err = msp_set_lds((const void *)k_tridiag_lp<NB, QMAX, RB, Q16, GP>, lds);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX, RB, Q16, GP>), dim3(batch), dim3(256), lds, s, n, H,
                        E, work);

@@ -372,6 +372,25 @@ class SpinEngine:
             g.R = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
             g.T2 = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
 
+    def execute_graph(self, plan):
+        """execute() replayed from a captured graph.  The first call runs the
+        plan once eagerly (kernel attributes, the side stream, buffers), then
+        captures one execute into a torch CUDA graph on a stream of default
+        priority; every call replays it.  Returns the results tensor, which
+        each replay overwrites; plan.checks refresh with it."""
+        if getattr(plan, "graph", None) is None:
+            self.execute(plan)
+            torch.cuda.synchronize(self.device)
+            g = torch.cuda.CUDAGraph()
+            cap = torch.cuda.Stream(self.device)
+            cap.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.graph(g, stream=cap, capture_error_mode="thread_local"):
+                res = self.execute(plan)
+            torch.cuda.current_stream(self.device).wait_stream(cap)
+            plan.graph, plan.graph_results = g, res
+        plan.graph.replay()
+        return plan.graph_results
+
     def execute(self, plan):
         """Launch every stage for every chunk; returns the device results
         tensor (n_slots, width).  Asynchronous on the current stream."""
