@@ -23,13 +23,16 @@ x axis in that order (simconfig.py:300-316); results have shape
 (store_time_slice, simconfig.py:347-368).
 """
 
+import datetime
+import os
+import re
 from collections import OrderedDict, namedtuple
 from itertools import product
 from numbers import Number
 
 import numpy as np
 
-from .orientations import normalised_weights, stored_orientations
+from .orientations import euler_zyz, normalised_weights, stored_orientations
 
 # keyword name -> snapshot field (simconfig.py:23-31)
 KEYWORD_FIELDS = OrderedDict([
@@ -58,16 +61,179 @@ def _flat_numbers(v):
         return False
 
 
+# coupling keyword -> (value shape, label) (simconfig.py:34-40,615-641)
+_COUPLINGS = {
+    "zeeman": ((3,), "Zeeman"),
+    "dipolar": ((3,), "dipolar"),
+    "hyperfine": ((3, 3), "hyperfine"),
+    "quadrupolar": ((3, 3), "quadrupolar"),
+    "dissipation": ((1,), "dissipation"),
+}
+
+
+def _spin_name(s):
+    """'2H' -> ('H', 2); other names unchanged (simconfig.py:528-537)."""
+    m = re.match("([0-9]+)([A-Z][a-z]*|e)", s)
+    return (m.group(2), int(m.group(1))) if m else str(s)
+
+
+def _celio_pair(words):
+    """(k, averages) of the celio keyword (simconfig.py:539-569)."""
+    out = []
+    for what, w in zip(("k", "averages"), words):
+        try:
+            v = int(w)
+        except ValueError:
+            raise MuSpinConfigError(
+                f"Value of {what} for Celio's method must be an integer") from None
+        if v < 0:
+            raise MuSpinConfigError(
+                f"Value of {what} for Celio's method must a postive integer or 0")
+        out.append(v)
+    out += [0] * (2 - len(out))
+    return out[0], out[1]
+
+
+def _entries(value):
+    """Entries of an evaluated keyword as flat float arrays."""
+    return [np.asarray(e, dtype=float).reshape(-1) for e in value]
+
+
+def _args_from_input(params, use_experimental_x_axis=True):
+    """Constructor arguments from MuSpinInput.evaluate's dictionary
+    (simconfig.py:82-321): name, spins and isotopes, axes, parameter ranges,
+    the fitting x axis, and the system built from the coupling blocks
+    (indices 1-based in the file)."""
+    from .spinsys import MuonSpinSystem
+    names = params["name"].value
+    if any(len(row) > 1 for row in names):
+        raise MuSpinConfigError("Name must be a word without spaces")
+    x_kw = params["x_axis"].value[0][0]
+    if x_kw not in KEYWORD_FIELDS:
+        raise MuSpinConfigError("Invalid x axis name in input file")
+    avg = []
+    for a in np.asarray(params["average_axes"].value).reshape(-1):
+        if a.lower() == "none":
+            continue
+        if a not in KEYWORD_FIELDS:
+            raise MuSpinConfigError("Invalid average axis name in input file")
+        avg.append(str(a))
+    for kw in KEYWORD_FIELDS:
+        if kw not in params:
+            raise MuSpinConfigError(
+                f"Invalid params object passed to MuSpinConfig: missing {kw}")
+    y_axis = str(params["y_axis"].value[0][0])
+    vals = {kw: list(params[kw].value) for kw in KEYWORD_FIELDS}
+    vals["time"] = []
+    for e in _entries(params["time"].value):
+        if e.size != 1:
+            raise MuSpinConfigError("Invalid line in time range")
+        vals["time"].append(float(e[0]))
+    vals["temperature"] = [float(e[0]) for e in _entries(params["temperature"].value)]
+    for kw in ("field", "intrinsic_field"):
+        vals[kw] = _entries(vals[kw])
+        for e in vals[kw]:
+            if e.size not in (1, 3):
+                raise MuSpinConfigError(
+                    "Invalid magnetic field value" if kw == "field" else
+                    "Invalid intrinsic magnetic field value")
+    vals["polarization"] = _entries(vals["polarization"])
+    vals["orientation"] = _entries(vals["orientation"])
+
+    finfo = params["fitting_info"]
+    if finfo["fit"]:
+        n = {kw: (1 if kw == "time" and y_axis == "integral" else len(v))
+             for kw, v in vals.items()}
+        if any(n[kw] > 1 for kw in KEYWORD_FIELDS if kw != x_kw and kw not in avg):
+            raise MuSpinConfigError("Can not have file ranges when fitting")
+        if use_experimental_x_axis or n[x_kw] <= 1:
+            xdata = np.asarray(finfo["data"], dtype=float)[:, 0]
+            if x_kw in ("field", "intrinsic_field"):
+                vals[x_kw] = [np.array([0.0, 0.0, v]) for v in xdata]
+            else:
+                vals[x_kw] = list(xdata)
+
+    system = MuonSpinSystem([_spin_name(s) for s in params["spins"].value[0]])
+    dissipation = {}
+    for c in params["couplings"].values():
+        if c.name not in _COUPLINGS:
+            raise MuSpinConfigError(
+                f"Invalid params object passed to MuSpinConfig: unknown {c.name} coupling")
+        idx = [v for v in c.args.values() if v is not None]
+        if any(v < 1 or v > len(system) for v in idx):
+            raise MuSpinConfigError("Out of range indices for coupling")
+        shape, label = _COUPLINGS[c.name]
+        v = np.asarray(c.value[0], dtype=float)
+        if v.shape != shape:
+            raise MuSpinConfigError(f"Invalid shape for {label} coupling term")
+        i = c.args["i"] - 1
+        j = c.args.get("j")
+        j = None if j is None else j - 1
+        if c.name == "zeeman":
+            system.add_zeeman_term(i, v)
+        elif c.name == "dipolar":
+            system.add_dipolar_term(i, j, v)
+        elif c.name == "hyperfine":
+            system.add_hyperfine_term(i, v, j)
+        elif c.name == "quadrupolar":
+            system.add_quadrupolar_term(i, v)
+        else:
+            dissipation[i] = float(v[0])
+
+    return dict(system=system, name=str(names[0][0]), field=vals["field"],
+                intrinsic_field=vals["intrinsic_field"], polarization=vals["polarization"],
+                time=vals["time"], orientation=vals["orientation"],
+                temperature=vals["temperature"], x_axis=x_kw, y_axis=y_axis,
+                average_axes=avg, dissipation=dissipation,
+                euler_mode=params["orientation"].args.get("mode", "zyz"),
+                results_function=params.get("results_function"),
+                celio=_celio_pair(params["celio"].value[0]))
+
+
+def _print_value(key, v):
+    """A file-range value as the .dat header shows it (simconfig.py:636-647)."""
+    if key in ("B", "intrinsic_B"):
+        return f"{v} T"
+    if key == "orient":
+        q, w = v
+        a, b, c = euler_zyz(q) * 180 / np.pi
+        return f"[ZYZ] a = {a:.1f} deg, b = {b:.1f} deg, c = {c:.1f} deg, weight = {w}"
+    return f"{v}"
+
+
 class MuSpinConfig:
-    def __init__(self, system, name="muspinsim", field=0.0, intrinsic_field=0.0,
+    def __init__(self, system=None, name="muspinsim", field=0.0, intrinsic_field=0.0,
                  polarization="transverse", time=None, orientation=((0.0, 0.0, 0.0),),
                  temperature=np.inf, x_axis="time", y_axis="asymmetry",
                  average_axes=("orientation",), dissipation=None, euler_mode="zyz",
-                 results_function=None):
+                 results_function=None, params=None, use_experimental_x_axis=True):
         """system: a MuonSpinSystem with its coupling terms; dissipation:
         {spin index: rate in MHz}; results_function: optional callable
-        f(x=..., y=..., **variables) applied to the results by run()."""
+        f(x=..., y=..., **variables) applied to the results by run().
+
+        Drop-in form: MuSpinConfig(params) with the dictionary returned by
+        MuSpinInput.evaluate (simconfig.py:82-321); an empty dictionary (or
+        no argument) gives an empty configuration.  use_experimental_x_axis
+        (fitting only): the x axis becomes the x column of the fitting data.
+        """
         from .spinsys import MuonSpinSystem
+        if isinstance(system, dict) and params is None:
+            params, system = system, None
+        self._constants = {}
+        self._file_ranges = OrderedDict()
+        self._avg_ranges = OrderedDict()
+        self._x_range = OrderedDict()
+        self._celio = (0, 0)
+        if params is not None:
+            if len(params) == 0:
+                return
+            args = _args_from_input(params, use_experimental_x_axis)
+            celio = args.pop("celio")
+            self.__init__(**args)
+            self._celio = celio
+            return
+        if system is None:
+            return
         if not isinstance(system, MuonSpinSystem):
             raise MuSpinConfigError("the system of a configuration must be a MuonSpinSystem")
         if y_axis not in ("asymmetry", "integral"):
@@ -232,8 +398,18 @@ class MuSpinConfig:
         return self._time_isavg
 
     @property
+    def spins(self):
+        return list(self._system.spins)
+
+    @property
     def celio_k(self):
-        return 0
+        """Trotter steps of Celio's method (0: exact evolution)."""
+        return self._celio[0]
+
+    @property
+    def celio_averages(self):
+        """Random-state averages of Celio's method (0: none)."""
+        return self._celio[1]
 
     def __len__(self):
         return len(self._configurations)
@@ -254,6 +430,31 @@ class MuSpinConfig:
         if isinstance(i, slice):
             return [self._snapshot(c) for c in self._configurations[i]]
         raise TypeError(f"Indices must be integer or slices, not {type(i)}")
+
+    def save_output(self, name=None, path=".", extension=".dat"):
+        """One text file per combination of file-range values, named
+        name[_i_j...]extension, with the columns x (|B| for a field axis) and
+        result, and a '#' header listing the file-range values
+        (simconfig.py:370-429)."""
+        from . import __version__
+        name = self.name if name is None else name
+        head = (f"MUSPINSIM v.{__version__}\nOutput file written on "
+                f"{datetime.datetime.now().ctime()}\nParameters used:\n")
+        x = self.x_axis_values
+        if self.x_axis in ("B", "intrinsic_B"):
+            x = np.linalg.norm(x, axis=-1)
+        elif self.x_axis == "orient":
+            x = np.arange(len(x))
+        keys = list(self._file_ranges)
+        written = []
+        for inds in product(*[range(len(v)) for v in self._file_ranges.values()]):
+            header = head + "".join(
+                "\t{0:<20} = {1}\n".format(k, _print_value(k, self._file_ranges[k][i]))
+                for k, i in zip(keys, inds))
+            fname = os.path.join(path, name + "".join(f"_{i}" for i in inds) + extension)
+            np.savetxt(fname, np.stack([x, self._results[inds]], axis=1), header=header)
+            written.append(fname)
+        return written
 
     def result_index(self, config_id):
         """Index of the results array a configuration contributes to."""

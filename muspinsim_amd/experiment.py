@@ -35,12 +35,21 @@ def _perp(z):
 
 
 class ExperimentRunner:
-    def __init__(self, config, variables=None):
-        """config: a MuSpinConfig; variables: values of fitting variables
-        (when given, run() leaves the results function to the caller, as in
-        experiment.py:60-70)."""
+    def __init__(self, config, variables=None, use_experimental_x_axis=True):
+        """config: a MuSpinConfig, or a MuSpinInput evaluated with `variables`
+        (experiment.py:25-70); variables: values of fitting variables (when
+        given, run() leaves the results function to the caller);
+        use_experimental_x_axis: for a fitting input, run on the x values of
+        the fitting data rather than the file's x axis."""
+        from .input import MuSpinInput
+        if isinstance(config, MuSpinInput):
+            config = MuSpinConfig(config.evaluate(**(variables or {})),
+                                  use_experimental_x_axis=use_experimental_x_axis)
         if not isinstance(config, MuSpinConfig):
-            raise TypeError("ExperimentRunner needs a MuSpinConfig")
+            raise TypeError("ExperimentRunner needs a MuSpinConfig or a MuSpinInput")
+        if config.celio_k > 0:
+            raise NotImplementedError("Celio's method (celio k > 0) is not provided by "
+                                      "this engine; it evolves exactly")
         self._config = config
         self._variables = {} if variables is None else dict(variables)
         self._system = config.system
@@ -266,7 +275,26 @@ class ExperimentRunner:
             res = np.mean(res, axis=-1) if cfg.time_averaged else res[..., 0]
         if cfg.x_axis == "t" and cfg.time_averaged:
             raise NotImplementedError("time can not be both the x axis and averaged")
-        if not self._variables and cfg.results_function is not None:
-            res = np.array(cfg.results_function(x=cfg.x_axis_values, y=res)).reshape(res.shape)
+        if not self._variables:
+            res = self.apply_results_function(res, {})
         cfg.results = res
         return cfg.results
+
+    def apply_results_function(self, results, variables):
+        """The configuration's results function of x (the x axis values), y
+        (the results) and the fitting variables, in the results' shape
+        (experiment.py:347-356).  It is a callable or a results_function
+        keyword of an input file; a scalar value fills the whole shape."""
+        f = self._config.results_function
+        if f is None:
+            return np.asarray(results)
+        x = self._config.x_axis_values
+        if hasattr(f, "evaluate"):
+            out = f.evaluate(**variables, x=x, y=results)
+        else:
+            out = f(x=x, y=results, **variables)
+        out = np.asarray(out)
+        shape = np.shape(results)
+        if out.size == int(np.prod(shape)):
+            return out.reshape(shape)
+        return np.array(np.broadcast_to(out.reshape(-1)[0] if out.size == 1 else out, shape))

@@ -1,7 +1,8 @@
 """FittingRunner of the drop-in surface (muspinsim/fitting.py:15-179).
 
 The reference evaluates its input file with the current variable values and
-runs an ExperimentRunner per objective evaluation; here a callable builds
+runs an ExperimentRunner per objective evaluation.  Here either an input
+file (FittingRunner(MuSpinInput), as in the reference) or a callable builds
 the MuSpinConfig from the experimental x values and the variables, and
 every evaluation runs the whole configuration in one batched GPU pass.
 
@@ -14,24 +15,44 @@ with the mean absolute deviation mean|y - y_target| as the scalar objective
 objective='chi2' selects sum((y - y_target) / sigma)^2 instead.
 """
 
+import os
 from numbers import Number
 
 import numpy as np
 from scipy.optimize import least_squares, minimize
 
+from .config import MuSpinConfig
 from .experiment import ExperimentRunner
 
 _METHODS = ("nelder-mead", "lbfgs", "least-squares")
 
 
 class FittingRunner:
-    def __init__(self, make_config, variables, data, method="nelder-mead", rtol=1e-3,
-                 objective="mean_abs", sigma=1.0, single_simulation=False):
+    def __init__(self, make_config, variables=None, data=None, method="nelder-mead",
+                 rtol=1e-3, objective="mean_abs", sigma=1.0, single_simulation=False):
         """make_config(x, **values) -> MuSpinConfig, where x holds the data's x
         values (the experimental x axis, simconfig.py:183-231); variables:
         {name: value} or {name: (value, lower, upper)}; data: (N, 2) array of
-        (x, y_target)."""
-        if len(variables) == 0:
+        (x, y_target).
+
+        Drop-in form: FittingRunner(MuSpinInput) takes the variables, data,
+        method, tolerance and single-simulation flag from the input file's
+        fitting keywords (fitting.py:15-64)."""
+        from .input import MuSpinInput
+        self._input = None
+        if isinstance(make_config, MuSpinInput):
+            inp = make_config
+            if len(inp.variables) == 0:
+                raise ValueError("MuSpinInput passed to FittingRunner has no variables to fit")
+            info = inp.fitting_info
+            self._input = inp
+            variables = {n: (v.value,) + tuple(v.bounds) for n, v in inp.variables.items()}
+            data, method, rtol = info["data"], info["method"], info["rtol"]
+            single_simulation = info["single_simulation"]
+
+            def make_config(x, **values):
+                return MuSpinConfig(inp.evaluate(**values))
+        if variables is None or len(variables) == 0:
             raise ValueError("FittingRunner needs at least one variable to fit")
         if method not in _METHODS:
             raise ValueError(f"fitting method must be one of {_METHODS}")
@@ -88,9 +109,7 @@ class FittingRunner:
             self.n_evaluations += 1
             if self._single:
                 self._cached = y
-        f = self._runner.config.results_function
-        if f is not None:
-            y = np.array(f(x=self._runner.config.x_axis_values, y=y, **vardict)).reshape(y.shape)
+        y = self._runner.apply_results_function(y, vardict)
         return np.asarray(y).reshape(-1)
 
     def residuals(self, x):
@@ -116,3 +135,54 @@ class FittingRunner:
                                  bounds=bounds)
         self._x = np.array(self._sol.x)
         return self._sol
+
+    def _final_runner(self, use_experimental_x_axis):
+        """A run at the solution, results function applied."""
+        vardict = dict(zip(self._xnames, self._x))
+        if self._input is not None:
+            runner = ExperimentRunner(self._input, variables=vardict,
+                                      use_experimental_x_axis=use_experimental_x_axis)
+        else:
+            runner = ExperimentRunner(self._make_config(self._xdata, **vardict),
+                                      variables=vardict)
+        y = runner.run()
+        runner.config.results = runner.apply_results_function(y, vardict)
+        return runner
+
+    def write_data(self, name=None, path="."):
+        """Save the fitted model as .dat files: on the input file's own x axis
+        when it has one, else on the data's x values (fitting.py:181-203)."""
+        if self._sol is None:
+            raise RuntimeError("run() the fit before writing its data")
+        self._final_runner(use_experimental_x_axis=False).config.save_output(name=name, path=path)
+
+    def write_report(self, fname=None, path="./"):
+        """Human-readable summary of the optimisation (fitting.py:205-249)."""
+        if self._sol is None:
+            raise RuntimeError("run() the fit before writing its report")
+        sol = self._sol
+        variables = dict(zip(self._xnames, sol.x))
+        if self._input is not None:
+            name = MuSpinConfig(self._input.evaluate(**variables)).name
+        else:
+            name = self._make_config(self._xdata, **variables).name
+        fname = fname if fname is not None else name + "_fit_report.txt"
+        err = sol.fun
+        if isinstance(err, np.ndarray):
+            err = float(np.average(np.abs(err)))
+        lines = [f"Fitting process for {name} completed",
+                 f"Success achieved: {sol.success}"]
+        if not sol.success:
+            lines.append(f"   Message: {sol.message}")
+        lines.append(f"Final absolute error <|f-f_targ|>: {err}")
+        n_sim = sol.nfev
+        if self._single:
+            n_sim = 1
+            lines.append(f"Number of 'results_function' evaluations: {sol.nfev}")
+        lines.append(f"Number of simulations: {n_sim}")
+        if self._method != "least-squares":
+            lines.append(f"Number of iterations: {sol.nit}")
+        lines += ["", "=" * 20, "", "Values found for fitting variables:", ""]
+        lines += [f"\t{k} = {v}" for k, v in variables.items()]
+        with open(os.path.join(path, fname), "w", encoding="utf-8") as fh:
+            fh.write("\n".join(lines) + "\n")

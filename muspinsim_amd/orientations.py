@@ -59,6 +59,23 @@ def from_polar(theta, phi):
     return from_euler(phi, theta, phi, "zyz")
 
 
+def euler_zyz(q):
+    """(a, b, c) with from_euler(a, b, c, 'zyz') = +-q: from the rotation
+    matrix R = Rz(c) Ry(b) Rz(a), b = arccos R22, a = atan2(R21, -R20),
+    c = atan2(R12, R02); at b = 0 or pi the split of a and c is not
+    determined and c = 0 is taken."""
+    w, x, y, z = np.asarray(q, dtype=float) / np.linalg.norm(q)
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    b = np.arccos(np.clip(R[2, 2], -1.0, 1.0))
+    if np.hypot(R[2, 0], R[2, 1]) > 1e-12:
+        return np.array([np.arctan2(R[2, 1], -R[2, 0]), b, np.arctan2(R[1, 2], R[0, 2])])
+    if R[2, 2] > 0:
+        return np.array([np.arctan2(R[1, 0], R[0, 0]), b, 0.0])
+    return np.array([np.arctan2(R[1, 0], R[1, 1]), b, 0.0])
+
+
 def rotate(q, v):
     """q v q^* for quaternions (N, 4) and vectors (N, 3) (or broadcastable)."""
     q = np.asarray(q, dtype=float)
