@@ -34,6 +34,33 @@ def _perp(z):
     return x, np.cross(z, x)
 
 
+def fold_results(cfg, out):
+    """Engine output (n_slots, width) -> the configuration's results shape:
+    slots x time when time is the x axis, else one value per slot (the
+    time average when time is an averaged axis)."""
+    out = np.asarray(out)
+    shape = cfg.results.shape
+    if cfg.x_axis == "t":
+        if cfg.time_averaged:
+            raise NotImplementedError("time can not be both the x axis and averaged")
+        return out.reshape(shape)
+    res = out.reshape(shape + (out.shape[1],))
+    return np.mean(res, axis=-1) if cfg.time_averaged else res[..., 0]
+
+
+def unfold_gradient(cfg, grad, width):
+    """Adjoint of fold_results: dL/d(results) -> dL/d(engine output)."""
+    grad = np.asarray(grad, dtype=float)
+    if cfg.x_axis == "t":
+        return grad.reshape(-1, width)
+    g = grad.reshape(-1, 1)
+    if cfg.time_averaged:
+        return np.repeat(g / width, width, axis=1)
+    out = np.zeros((g.shape[0], width))
+    out[:, :1] = g
+    return out
+
+
 class ExperimentRunner:
     def __init__(self, config, variables=None, use_experimental_x_axis=True):
         """config: a MuSpinConfig, or a MuSpinInput evaluated with `variables`
@@ -267,14 +294,7 @@ class ExperimentRunner:
         if self._engine is None:
             self._engine = SpinEngine("cuda:0")
         out = self._engine.run(systems, batch)          # (n_slots, width)
-        shape = cfg.results.shape
-        if cfg.x_axis == "t":
-            res = out.reshape(shape)
-        else:
-            res = out.reshape(shape + (out.shape[1],))
-            res = np.mean(res, axis=-1) if cfg.time_averaged else res[..., 0]
-        if cfg.x_axis == "t" and cfg.time_averaged:
-            raise NotImplementedError("time can not be both the x axis and averaged")
+        res = fold_results(cfg, out)
         if not self._variables:
             res = self.apply_results_function(res, {})
         cfg.results = res
