@@ -227,6 +227,34 @@ int msp_trace_products(int n, int nparam, const double *d_G, const double *d_dH,
 int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d_idx,
                    const double *d_rows, double scale, double *d_results, void *stream);
 
+/* Celio's Trotter evolution of nstates state vectors of dimension n
+ * (replaces cpp/celio.cpp:23-68 with cpp/parallel.cpp:130-266, the loop of
+ * CelioHamiltonian._fast_evolve_cpp, celio.py:431-476; the density-matrix
+ * CelioHamiltonian.evolve, celio.py:210-293, maps onto it as a weighted sum
+ * over product states).  Each state s belongs to configuration
+ * c = d_state_cfg[s]; every time step writes d_out[s*ntimes + t] =
+ * Re <psi|O_c|psi> and then applies k times the Trotter factors
+ * d_desc[first_c .. first_c + count_c) with (first_c, count_c) =
+ * d_cfg_steps[2c], d_cfg_steps[2c+1].  A descriptor is four int32
+ * (m, matrix offset, fiber-table offset, element-table offset): the m x m
+ * row-major complex matrix at d_mats + 2*offset acts on the fibers whose
+ * base indices are d_tables[fiber .. fiber + n/m) through the m element
+ * offsets d_tables[element .. element + m).  d_meas[4c..4c+3] describes O_c
+ * (m = 2, the muon).  init_mode 0: psi = mu (x) exp(2 pi i phi)/sqrt(n/2)
+ * with the muon first, mu = the 2 complex at d_init_vec + 2*d_init_idx[s],
+ * phi = d_phases[s*(n/2) ..]; init_mode 1: product state
+ * psi[I] = prod_q v_q[digit_q(I)] with v_q at d_init_vec + 2*d_init_idx[s*n_spins + q].
+ * n > 8192 keeps the states in d_work (msp_celio_workspace bytes); smaller
+ * n keeps each state in LDS and needs no workspace.  Factor sizes m: any
+ * m <= 64. */
+size_t msp_celio_workspace(int n, int nstates);
+int msp_celio_evolve(int n, int n_spins, const int32_t *h_dims, int nstates, int ntimes, int k,
+                     const int32_t *d_state_cfg, const int32_t *d_cfg_steps,
+                     const int32_t *d_desc, const int32_t *d_meas, const double *d_mats,
+                     const int32_t *d_tables, int init_mode, const double *d_init_vec,
+                     const int32_t *d_init_idx, const double *d_phases, double *d_out,
+                     double *d_work, size_t work_bytes, void *stream);
+
 /* Graph capture and replay of a sequence of msp_* calls on one stream (no
  * reference counterpart: the reference re-runs Python per instance).  Between
  * msp_graph_begin and msp_graph_end the calls on `stream` are recorded, not

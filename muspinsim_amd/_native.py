@@ -72,6 +72,9 @@ SIGNATURES = {
     "msp_batch_sum": (_I, [_I, _I, _P, _P, _P]),
     "msp_trace_products": (_I, [_I, _I, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
+    "msp_celio_workspace": (_SZ, [_I, _I]),
+    "msp_celio_evolve": (_I, [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P,
+                              _P, _P, _SZ, _P]),
     "msp_graph_begin": (_I, [_P]),
     "msp_graph_end": (_I, [_P, ctypes.POINTER(_P)]),
     "msp_graph_launch": (_I, [_P, _P]),

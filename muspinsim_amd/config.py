@@ -153,7 +153,8 @@ def _args_from_input(params, use_experimental_x_axis=True):
             else:
                 vals[x_kw] = list(xdata)
 
-    system = MuonSpinSystem([_spin_name(s) for s in params["spins"].value[0]])
+    celio = _celio_pair(params["celio"].value[0])
+    system = MuonSpinSystem([_spin_name(s) for s in params["spins"].value[0]], celio[0])
     dissipation = {}
     for c in params["couplings"].values():
         if c.name not in _COUPLINGS:
@@ -187,7 +188,7 @@ def _args_from_input(params, use_experimental_x_axis=True):
                 average_axes=avg, dissipation=dissipation,
                 euler_mode=params["orientation"].args.get("mode", "zyz"),
                 results_function=params.get("results_function"),
-                celio=_celio_pair(params["celio"].value[0]))
+                celio=celio)
 
 
 def _print_value(key, v):

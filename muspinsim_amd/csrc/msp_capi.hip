@@ -412,6 +412,48 @@ int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d
                       "msp_accumulate");
 }
 
+size_t msp_celio_workspace(int n, int nstates)
+{
+    if (n < 1 || nstates < 0) return 0;
+    return celio_workspace_bytes(n, nstates);
+}
+
+int msp_celio_evolve(int n, int n_spins, const int32_t *h_dims, int nstates, int ntimes, int k,
+                     const int32_t *d_state_cfg, const int32_t *d_cfg_steps,
+                     const int32_t *d_desc, const int32_t *d_meas, const double *d_mats,
+                     const int32_t *d_tables, int init_mode, const double *d_init_vec,
+                     const int32_t *d_init_idx, const double *d_phases, double *d_out,
+                     double *d_work, size_t work_bytes, void *stream)
+{
+    if (n_spins < 1 || n_spins > MSP_MAX_SPINS || !h_dims)
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: n_spins = %d out of range", n_spins);
+    long long prod = 1;
+    for (int q = 0; q < n_spins; ++q) {
+        if (h_dims[q] < 1) return fail(MSP_ERR_VALUE, "msp_celio_evolve: bad spin dimension");
+        prod *= h_dims[q];
+    }
+    if (n < 2 || prod != n || n > (1 << 26))
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: n = %d does not match the spin dimensions", n);
+    if (nstates < 0 || ntimes < 0 || k < 1)
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: bad nstates / ntimes / k");
+    if (nstates == 0 || ntimes == 0) return MSP_OK;
+    if (init_mode != 0 && init_mode != 1)
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: init_mode must be 0 or 1");
+    if (init_mode == 0 && (h_dims[0] != 2 || !d_phases))
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: random states need the muon first and phases");
+    if (!d_state_cfg || !d_cfg_steps || !d_desc || !d_meas || !d_mats || !d_tables ||
+        !d_init_vec || !d_init_idx || !d_out)
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: null pointer");
+    const size_t need = celio_workspace_bytes(n, nstates);
+    if (need > 0 && (!d_work || work_bytes < need))
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: workspace of %zu bytes needed", need);
+    return hip_status(launch_celio_evolve(n, n_spins, h_dims, nstates, ntimes, k, d_state_cfg,
+                                          d_cfg_steps, d_desc, d_meas, d_mats, d_tables, init_mode,
+                                          d_init_vec, d_init_idx, d_phases, d_out, d_work,
+                                          S(stream)),
+                      "msp_celio_evolve");
+}
+
 int msp_profile_enable(int on)
 {
     prof_enable(on);

@@ -96,3 +96,13 @@ hipError_t launch_trace_products(int n, int nparam, const double *G, const doubl
 // Batched complex C = A^H B on FP64 matrix cores (msp_gemm_mfma.hip)
 hipError_t launch_gemm_cn_mfma(int n, int batch, const double *A, const double *B, double *C,
                                hipStream_t s);
+
+// Celio/Trotter evolution (msp_celio.hip)
+int celio_block_threads(int n);
+size_t celio_workspace_bytes(int n, int nstates);
+hipError_t launch_celio_evolve(int n, int n_spins, const int32_t *dims, int nstates, int ntimes,
+                               int k, const int32_t *state_cfg, const int32_t *cfg_steps,
+                               const int32_t *desc, const int32_t *meas, const double *mats,
+                               const int32_t *tables, int init_mode, const double *init_vec,
+                               const int32_t *init_idx, const double *phases, double *out,
+                               double *work, hipStream_t s);

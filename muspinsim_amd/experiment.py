@@ -19,7 +19,7 @@ from .engine import InstanceBatch, SpinEngine
 from .hamiltonian import Hamiltonian
 from .lindbladian import Lindbladian
 from .orientations import rotate
-from .spinop import DensityOperator, SpinOperator
+from .spinop import DensityOperator, SpinOperator, spin_matrices
 
 
 def _perp(z):
@@ -32,6 +32,20 @@ def _perp(z):
         return np.array([1.0, 0, 0]), np.array([0, z[2], 0])
     x = np.array([z[1], -z[0], 0.0]) / np.hypot(z[0], z[1])
     return x, np.cross(z, x)
+
+
+class _Term:
+    """A linear term as Celio's splitting reads it (indices and tensor)."""
+
+    def __init__(self, indices, tensor):
+        self.indices = np.array(indices, dtype=int)
+        self.tensor = np.asarray(tensor, dtype=float)
+
+
+def celio_plus(op):
+    """Eigenvector of the largest eigenvalue of a Hermitian matrix."""
+    w, v = np.linalg.eigh(np.asarray(op, dtype=complex))
+    return v[:, int(np.argmax(w))]
 
 
 def fold_results(cfg, out):
@@ -74,9 +88,6 @@ class ExperimentRunner:
                                   use_experimental_x_axis=use_experimental_x_axis)
         if not isinstance(config, MuSpinConfig):
             raise TypeError("ExperimentRunner needs a MuSpinConfig or a MuSpinInput")
-        if config.celio_k > 0:
-            raise NotImplementedError("Celio's method (celio k > 0) is not provided by "
-                                      "this engine; it evolves exactly")
         self._config = config
         self._variables = {} if variables is None else dict(variables)
         self._system = config.system
@@ -237,6 +248,8 @@ class ExperimentRunner:
 
     def run_single(self, cfg_snap):
         """One configuration through the operator classes (experiment.py:434-498)."""
+        if self._config.celio_k > 0:
+            return self._celio_rows([cfg_snap])[0]
         w = self.load_config(cfg_snap)
         H = self.Htot
         if cfg_snap.y == "asymmetry":
@@ -285,11 +298,94 @@ class ExperimentRunner:
                               n_slots=n_slots, avg_N=cfg.avg_N)
         return [desc], batch
 
+    # ---------------------------------------------------------- Celio
+    def _celio_rows(self, snaps):
+        """Weighted time series of configurations under Celio's method
+        (experiment.py:238-269,451-472): random-phase states when
+        celio_averages > 0 (T = inf, muon first), else the density matrix as
+        a weighted sum of product states; all configurations in one launch."""
+        from . import celio
+        cfg = self._config
+        s = self._system
+        if cfg.dissipation_terms:
+            raise NotImplementedError("Dissipation is not supported when using Celio's method")
+        if cfg.y_axis == "integral":
+            raise NotImplementedError("integrate_decaying is not implemented for Celio's method")
+        times = np.atleast_1d(np.array(snaps[0].t, dtype=float))
+        avg = cfg.celio_averages
+        dt = times[1] - times[0] if times.size > 1 else 0.0
+        job = celio.CelioJob(s.Is, dt, cfg.celio_k)
+        Sm = spin_matrices(0.5)
+        combine = []
+        for snap in snaps:
+            w = self.load_config(snap)
+            terms = list(s.terms)
+            if not np.array_equal(self._B, [0, 0, 0]):
+                terms += [_Term([i], self._B * g) for i, g in enumerate(s.gammas)]
+            celio.validate(terms, times)
+            contribs = celio.h_contribs(s.Is, terms)
+            if avg > 0:
+                if self._T != np.inf:
+                    raise ValueError("The fast version of Celio's method requires T -> inf. "
+                                     "Either remove the number of averages or don't use celio.")
+                if s.muon_index != 0:
+                    raise ValueError("Muon must be the first spin in the system in order to "
+                                     "use the fast Celio method")
+                sig = sum(self._p[a] * 2.0 * Sm[a] for a in range(3))
+                c = job.add_config(contribs, sig, 0)
+                job.add_random_states(c, celio.plus_state(sig), avg)
+                combine.append((np.full(avg, 0.5 / avg), w))
+            else:
+                c = job.add_config(contribs, sum(self._p[a] * Sm[a] for a in range(3)),
+                                   s.muon_index, dagger=True)
+                combine.append((np.array(job.add_product_states(c, self._rho0_factors())), w))
+        out = job.run(times.size)
+        rows, s0 = [], 0
+        for wts, w in combine:
+            rows.append((wts @ out[s0:s0 + len(wts)]) * w)
+            s0 += len(wts)
+        return np.array(rows)
+
+    def _rho0_factors(self):
+        """Per-spin (eigenvectors, weights) of the initial state: the muon
+        pure along p, the others thermal under their Zeeman term (the factors
+        of the rho0 property)."""
+        out = []
+        for i in range(len(self._system)):
+            I = self._system.I(i)
+            S = spin_matrices(I)
+            if i == self._system.muon_index:
+                out.append((celio_plus(sum(self._p[a] * S[a] for a in range(3)))[:, None],
+                            np.ones(1)))
+                continue
+            Hz = sum(self._B[a] * S[a] for a in range(3))
+            E, V = np.linalg.eigh(Hz)
+            E = E * 1e6 * self._system.gamma(i)
+            if self._T > 0:
+                Z = np.exp(-cnst.h * E / (cnst.k * self._T))
+            else:
+                Z = np.where(E == np.min(E), 1.0, 0.0)
+            Z = Z / np.sum(Z) if np.sum(Z) > 0 else np.ones(len(E)) / len(E)
+            out.append((V, Z))
+        return out
+
     def run(self):
         """Every configuration on the GPU; results (file ranges..., x), with
         the results function applied when no fitting variables are set
         (experiment.py:358-382)."""
         cfg = self._config
+        if cfg.celio_k > 0:
+            snaps = cfg[0:len(cfg)]
+            rows = self._celio_rows(snaps)
+            cfg.results = np.zeros(cfg.results.shape)
+            scalar = cfg.x_axis != "t" and not cfg.time_averaged
+            for snap, row in zip(snaps, rows):
+                cfg.store_time_slice(snap.id, float(row[0]) if scalar else row)
+            res = cfg.results
+            if not self._variables:
+                res = self.apply_results_function(res, {})
+            cfg.results = res
+            return cfg.results
         systems, batch = self.instance_batch()
         if self._engine is None:
             self._engine = SpinEngine("cuda:0")

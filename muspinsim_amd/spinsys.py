@@ -142,10 +142,10 @@ class DissipationTerm:
 class SpinSystem:
     def __init__(self, spins=None, celio_k=0):
         """Spins as 'e', 'mu', an element symbol, or (symbol, mass number)
-        (spinsys.py:161-216).  celio_k > 0 (Trotter/Celio evolution) is not
-        part of this engine and raises NotImplementedError."""
-        if celio_k:
-            raise NotImplementedError("Celio's method is not provided by this engine")
+        (spinsys.py:161-216).  celio_k > 0: the hamiltonian is a
+        CelioHamiltonian with that Trotter factor."""
+        if int(celio_k) < 0:
+            raise ValueError("celio_k must be a non-negative integer")
         spins = [] if spins is None else spins
         gammas, Qs, Is = [], [], []
         for s in spins:
@@ -161,7 +161,7 @@ class SpinSystem:
         self._operators = [{a: SpinOperator.from_axes(I, a) for a in "xyz+-0"} for I in self._Is]
         self._terms = []
         self._dissip_terms = []
-        self._celio_k = 0
+        self._celio_k = int(celio_k)
 
     def clone(self):
         ans = copy.copy(self)
@@ -342,13 +342,19 @@ class SpinSystem:
 
     @property
     def hamiltonian(self):
-        """Hamiltonian (MHz) of all terms, built on the GPU (spinsys.py:613-626)."""
+        """Hamiltonian (MHz) of all terms, built on the GPU, or the
+        CelioHamiltonian of the terms when celio_k > 0 (spinsys.py:613-626)."""
+        if self._celio_k:
+            from .celio import CelioHamiltonian
+            return CelioHamiltonian(self._terms, self._celio_k, self)
         from . import gpu
         from .hamiltonian import Hamiltonian
         return Hamiltonian(gpu.build_h(self.to_desc()), dim=self.dimension)
 
     @property
     def lindbladian(self):
+        if self._celio_k:
+            raise NotImplementedError("Dissipation is not supported when using Celio's method")
         from .lindbladian import Lindbladian
         return Lindbladian.from_hamiltonian(self.hamiltonian,
                                             [t.tuple for t in self._dissip_terms])
