@@ -256,15 +256,13 @@ class CelioJob:
     def nstates(self):
         return len(self._state_cfg)
 
-    def run(self, ntimes, engine=None):
-        """(nstates, ntimes) Re <psi_s(t)| O |psi_s(t)> on the GPU."""
+    def prepare(self, engine=None):
+        """Upload the tables, descriptors and initial-state data and compute
+        the factor matrices on the device; returns the launch arguments."""
         from . import gpu
         eng = engine or gpu.engine()
         dev = eng.device
-        st = eng.stream()
-        if self.nstates == 0 or ntimes == 0:
-            return np.zeros((self.nstates, ntimes))
-        mats = torch.zeros((self._nmat,), dtype=torch.complex128, device=dev)
+        mats = torch.zeros((max(self._nmat, 1),), dtype=torch.complex128, device=dev)
         for off, M in self._fixed:
             mats[off:off + M.size] = torch.as_tensor(M.reshape(-1), device=dev)
         by_m = {}
@@ -280,30 +278,62 @@ class CelioJob:
             mats[idx.reshape(-1)] = C.reshape(-1)
         tables = np.concatenate(self._tables).astype(np.int32)
         for (m, _, fib, el) in self._desc + self._meas:
-            assert (fib + self.n // m <= tables.size and el + m <= tables.size)
-        desc = torch.as_tensor(np.array(self._desc, dtype=np.int32).reshape(-1), device=dev)
-        meas = torch.as_tensor(np.array(self._meas, dtype=np.int32).reshape(-1), device=dev)
-        steps = torch.as_tensor(np.array(self._cfg_steps, dtype=np.int32).reshape(-1), device=dev)
-        tab = torch.as_tensor(tables, device=dev)
-        scfg = torch.as_tensor(np.array(self._state_cfg, dtype=np.int32), device=dev)
-        ivec = torch.as_tensor(np.concatenate(self._init_vec), device=dev)
-        iidx = torch.as_tensor(np.array(self._init_idx, dtype=np.int32).reshape(-1), device=dev)
-        if self.mode == 0:
-            phases = torch.as_tensor(np.stack(self._phases), device=dev)
-        else:
-            phases = None
-        out = torch.empty((self.nstates, ntimes), dtype=torch.float64, device=dev)
+            if fib + self.n // m > tables.size or el + m > tables.size:
+                raise RuntimeError("Celio index tables are inconsistent")
+
+        def i32(a):
+            return torch.as_tensor(np.array(a, dtype=np.int32).reshape(-1), device=dev)
+        prep = {
+            "eng": eng, "mats": mats, "tables": torch.as_tensor(tables, device=dev),
+            "desc": i32(self._desc), "meas": i32(self._meas), "steps": i32(self._cfg_steps),
+            "state_cfg": i32(self._state_cfg),
+            "init_vec": torch.as_tensor(np.concatenate(self._init_vec), device=dev),
+            "init_idx": i32(self._init_idx),
+            "phases": (torch.as_tensor(np.stack(self._phases), device=dev)
+                       if self.mode == 0 else None),
+        }
         wsb = N.load_library().msp_celio_workspace(self.n, self.nstates)
-        work = torch.empty((max(wsb, 8) // 8,), dtype=torch.float64, device=dev) if wsb else None
+        prep["work"] = (torch.empty((max(wsb, 8) // 8,), dtype=torch.float64, device=dev)
+                        if wsb else None)
+        prep["wsb"] = int(wsb)
+        return prep
+
+    def launch(self, prep, ntimes, out=None):
+        """One msp_celio_evolve over every state; (nstates, ntimes) device
+        tensor of Re <psi_s(t)| O |psi_s(t)>."""
         from .engine import ctypes_int_array
+        eng = prep["eng"]
+        if out is None:
+            out = torch.empty((self.nstates, ntimes), dtype=torch.float64, device=eng.device)
+        if self.nstates == 0 or ntimes == 0:
+            return out
+        ph = prep["phases"]
+        wk = prep["work"]
         N.check(eng.lib.msp_celio_evolve(
             self.n, len(self.dims), ctypes_int_array(self.dims), self.nstates, int(ntimes), self.k,
-            scfg.data_ptr(), steps.data_ptr(), desc.data_ptr(), meas.data_ptr(),
-            torch.view_as_real(mats).data_ptr(), tab.data_ptr(), int(self.mode),
-            torch.view_as_real(ivec).data_ptr(), iidx.data_ptr(),
-            None if phases is None else phases.data_ptr(), out.data_ptr(),
-            None if work is None else work.data_ptr(), int(wsb), st))
-        return out.cpu().numpy()
+            prep["state_cfg"].data_ptr(), prep["steps"].data_ptr(), prep["desc"].data_ptr(),
+            prep["meas"].data_ptr(), torch.view_as_real(prep["mats"]).data_ptr(),
+            prep["tables"].data_ptr(), int(self.mode),
+            torch.view_as_real(prep["init_vec"]).data_ptr(), prep["init_idx"].data_ptr(),
+            None if ph is None else ph.data_ptr(), out.data_ptr(),
+            None if wk is None else wk.data_ptr(), prep["wsb"], eng.stream()))
+        return out
+
+    def flops_per_state(self, ntimes):
+        """Real flops of one state's evolution (complex multiply-add = 8):
+        8 n m per factor application, k applications of each per step."""
+        per_cfg = []
+        for first, count in self._cfg_steps:
+            per_cfg.append(sum(8.0 * self.n * self._desc[first + c][0] for c in range(count)))
+        per_state = np.array([per_cfg[c] for c in self._state_cfg])
+        return float(per_state.mean()) * self.k * max(ntimes - 1, 0)
+
+    def run(self, ntimes, engine=None):
+        """(nstates, ntimes) Re <psi_s(t)| O |psi_s(t)> on the GPU."""
+        if self.nstates == 0 or ntimes == 0:
+            return np.zeros((self.nstates, ntimes))
+        prep = self.prepare(engine)
+        return self.launch(prep, ntimes).cpu().numpy()
 
 
 def plus_state(sigma):

@@ -207,10 +207,12 @@ class MuSpinConfig:
                  polarization="transverse", time=None, orientation=((0.0, 0.0, 0.0),),
                  temperature=np.inf, x_axis="time", y_axis="asymmetry",
                  average_axes=("orientation",), dissipation=None, euler_mode="zyz",
-                 results_function=None, params=None, use_experimental_x_axis=True):
+                 results_function=None, celio=(0, 0), params=None,
+                 use_experimental_x_axis=True):
         """system: a MuonSpinSystem with its coupling terms; dissipation:
         {spin index: rate in MHz}; results_function: optional callable
-        f(x=..., y=..., **variables) applied to the results by run().
+        f(x=..., y=..., **variables) applied to the results by run();
+        celio: (k, averages) of Celio's method, (0, 0) for exact evolution.
 
         Drop-in form: MuSpinConfig(params) with the dictionary returned by
         MuSpinInput.evaluate (simconfig.py:82-321); an empty dictionary (or
@@ -228,10 +230,7 @@ class MuSpinConfig:
         if params is not None:
             if len(params) == 0:
                 return
-            args = _args_from_input(params, use_experimental_x_axis)
-            celio = args.pop("celio")
-            self.__init__(**args)
-            self._celio = celio
+            self.__init__(**_args_from_input(params, use_experimental_x_axis))
             return
         if system is None:
             return
@@ -245,6 +244,9 @@ class MuSpinConfig:
         for a in avg:
             if a not in KEYWORD_FIELDS:
                 raise MuSpinConfigError("Invalid average axis name")
+        if len(celio) != 2 or any(int(c) != c or c < 0 for c in celio):
+            raise MuSpinConfigError("celio must be (k, averages), non-negative integers")
+        self._celio = (int(celio[0]), int(celio[1]))
         self._name = name
         self._system = system
         self._y_axis = y_axis

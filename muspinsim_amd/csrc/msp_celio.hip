@@ -27,6 +27,7 @@
 namespace {
 
 constexpr int kLdsStateMax = 8192;
+constexpr int kRedSlots = 8;    // reduction slots (one per wavefront, 512 threads)
 
 struct CelioDims {
     int n_spins;
@@ -40,7 +41,7 @@ __device__ inline double2 cmul(double2 a, double2 b)
 
 // <x|O|x> over the fibers of the muon (M = 2), this thread's part
 template <int M>
-__device__ double measure_part(const double2 *psi, int n, int4 md, const double2 *__restrict__ mats,
+__device__ __attribute__((always_inline)) double measure_part(const double2 *psi, int n, int4 md, const double2 *__restrict__ mats,
                                const int *__restrict__ tables)
 {
     const int nf = n / M;
@@ -71,18 +72,18 @@ __device__ double measure_part(const double2 *psi, int n, int4 md, const double2
 
 // one thread per fiber, the fiber in registers (compile-time M)
 template <int M>
-__device__ void apply_reg(double2 *psi, int n, int4 d, const double2 *__restrict__ mats,
-                          const int *__restrict__ tables)
+__device__ __attribute__((always_inline)) void apply_reg(double2 *psi, int n, int4 d,
+                                                         const double2 *Mx,
+                                                         const int *__restrict__ tables)
 {
     const int nf = n / M;
-    const double2 *Mx = mats + d.y;
     const int *fib = tables + d.z;
     const int *el = tables + d.w;
     int off[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) off[j] = el[j];
     for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-        // keeps the matrix loads inside the loop: hoisted, its M^2 entries
+        // keeps the matrix reads inside the loop: hoisted, its M^2 entries
         // would be held in registers across fibers
         __asm__ volatile("" ::: "memory");
         const int base = fib[f];
@@ -103,7 +104,7 @@ __device__ void apply_reg(double2 *psi, int n, int4 d, const double2 *__restrict
     }
 }
 
-__device__ void apply_rows(double2 *psi, int n, int4 d, const double2 *__restrict__ mats,
+__device__ __attribute__((always_inline)) void apply_rows(double2 *psi, int n, int4 d, const double2 *__restrict__ mats,
                            const int *__restrict__ tables)
 {
     const int m = d.x;
@@ -136,7 +137,7 @@ __device__ void apply_rows(double2 *psi, int n, int4 d, const double2 *__restric
     }
 }
 
-__device__ double block_sum(double v, double *red)
+__device__ __attribute__((always_inline)) double block_sum(double v, double *red)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -202,20 +203,23 @@ __global__ __launch_bounds__(512) void k_celio_evolve(
         if (t == ntimes - 1) break;
         for (int r = 0; r < k; ++r)
             for (int c = 0; c < count; ++c) {
+                // the descriptor and the matrix address are uniform, so the
+                // matrix entries are read through the scalar cache
                 const int4 d = desc[first + c];
+                const double2 *cur = mats + d.y;
                 switch (d.x) {
-                case 2: apply_reg<2>(psi, n, d, mats, tables); break;
-                case 3: apply_reg<3>(psi, n, d, mats, tables); break;
-                case 4: apply_reg<4>(psi, n, d, mats, tables); break;
-                case 5: apply_reg<5>(psi, n, d, mats, tables); break;
-                case 6: apply_reg<6>(psi, n, d, mats, tables); break;
-                case 7: apply_reg<7>(psi, n, d, mats, tables); break;
-                case 8: apply_reg<8>(psi, n, d, mats, tables); break;
-                case 9: apply_reg<9>(psi, n, d, mats, tables); break;
-                case 10: apply_reg<10>(psi, n, d, mats, tables); break;
-                case 12: apply_reg<12>(psi, n, d, mats, tables); break;
-                case 14: apply_reg<14>(psi, n, d, mats, tables); break;
-                case 16: apply_reg<16>(psi, n, d, mats, tables); break;
+                case 2: apply_reg<2>(psi, n, d, cur, tables); break;
+                case 3: apply_reg<3>(psi, n, d, cur, tables); break;
+                case 4: apply_reg<4>(psi, n, d, cur, tables); break;
+                case 5: apply_reg<5>(psi, n, d, cur, tables); break;
+                case 6: apply_reg<6>(psi, n, d, cur, tables); break;
+                case 7: apply_reg<7>(psi, n, d, cur, tables); break;
+                case 8: apply_reg<8>(psi, n, d, cur, tables); break;
+                case 9: apply_reg<9>(psi, n, d, cur, tables); break;
+                case 10: apply_reg<10>(psi, n, d, cur, tables); break;
+                case 12: apply_reg<12>(psi, n, d, cur, tables); break;
+                case 14: apply_reg<14>(psi, n, d, cur, tables); break;
+                case 16: apply_reg<16>(psi, n, d, cur, tables); break;
                 default: apply_rows(psi, n, d, mats, tables); break;
                 }
                 __syncthreads();
@@ -249,9 +253,9 @@ hipError_t launch_celio_evolve(int n, int n_spins, const int32_t *dims, int nsta
     cd.n_spins = n_spins;
     for (int q = 0; q < n_spins; ++q) cd.d[q] = dims[q];
     const int bs = celio_block_threads(n);
-    const size_t red = (size_t)(bs / 64) * sizeof(double);
+    const size_t extra = kRedSlots * sizeof(double);     // reduction slots
     const bool global = n > kLdsStateMax;
-    const size_t lds = global ? red : (size_t)n * 16 + red;
+    const size_t lds = global ? extra : (size_t)n * 16 + extra;
     const void *f = global ? (const void *)k_celio_evolve<true> : (const void *)k_celio_evolve<false>;
     hipError_t e = msp_set_lds(f, lds);
     if (e != hipSuccess) return e;

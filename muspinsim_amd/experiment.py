@@ -304,6 +304,17 @@ class ExperimentRunner:
         (experiment.py:238-269,451-472): random-phase states when
         celio_averages > 0 (T = inf, muon first), else the density matrix as
         a weighted sum of product states; all configurations in one launch."""
+        job, combine, ntimes = self._celio_job(snaps)
+        out = job.run(ntimes)
+        rows, s0 = [], 0
+        for wts, w in combine:
+            rows.append((wts @ out[s0:s0 + len(wts)]) * w)
+            s0 += len(wts)
+        return np.array(rows)
+
+    def _celio_job(self, snaps):
+        """(CelioJob of the configurations, [(state weights, orientation
+        weight)] per configuration, number of time points)."""
         from . import celio
         cfg = self._config
         s = self._system
@@ -339,12 +350,7 @@ class ExperimentRunner:
                 c = job.add_config(contribs, sum(self._p[a] * Sm[a] for a in range(3)),
                                    s.muon_index, dagger=True)
                 combine.append((np.array(job.add_product_states(c, self._rho0_factors())), w))
-        out = job.run(times.size)
-        rows, s0 = [], 0
-        for wts, w in combine:
-            rows.append((wts @ out[s0:s0 + len(wts)]) * w)
-            s0 += len(wts)
-        return np.array(rows)
+        return job, combine, times.size
 
     def _rho0_factors(self):
         """Per-spin (eigenvectors, weights) of the initial state: the muon

@@ -272,3 +272,33 @@ def c5_fit(seed=5, nori=256, ntimes=2000):
     sysd, _ = c5_system(seed=seed)
     times = np.linspace(0.0, 10.0, ntimes)
     return [sysd], powder_batch(nori, [[0, 0, 0]], times, "asymmetry", [0, 0, 1.0])
+
+
+def celio_mu4v_config(nori=256, averages=4, ntimes=50, k=10, field=0.001):
+    """Celio workload: a muon among four 51V (I = 7/2, n = 8192) with
+    muon-V dipolar couplings and V quadrupole tensors, a small longitudinal
+    field so that the crystallite orientation matters, ZCW powder of at
+    least `nori` orientations, `averages` random-phase states per
+    orientation, times 0..10 us; the shape of the reference's celio example
+    (examples/celio/celio.in: mu + 4 V, celio 10 4, 50 times) on a powder."""
+    from .config import MuSpinConfig
+    from .orientations import zcw
+    from .spinsys import MuonSpinSystem
+    s = MuonSpinSystem(["mu", "V", "V", "V", "V"], celio_k=k)
+    r = [[0.0, -1.3, 1.2], [-1.25, 0.0, -1.22], [1.24, 0.0, -1.21], [0.0, 1.26, 1.23]]
+    efg = [[[-0.031, 0, 0], [0, -0.009, -0.012], [0, -0.012, 0.040]],
+           [[-0.009, 0, 0.012], [0, -0.030, 0], [0.012, 0, 0.039]],
+           [[-0.009, 0, -0.012], [0, -0.031, 0], [-0.012, 0, 0.040]],
+           [[-0.031, 0, 0], [0, -0.009, 0.012], [0, 0.012, 0.040]]]
+    for j in range(4):
+        s.add_bilinear_term(0, j + 1, _dip(r[j]), "Dipolar")
+        s.add_bilinear_term(j + 1, j + 1, np.array(efg[j]), "Quadrupolar")
+    return MuSpinConfig(s, field=field, orientation=zcw(nori), polarization="longitudinal",
+                        time=np.linspace(0.0, 10.0, ntimes), celio=(k, averages))
+
+
+def _dip(v):
+    """A dipolar-like traceless tensor from a vector (MHz scale ~ |v|)."""
+    v = np.asarray(v, dtype=float)
+    u = v / np.linalg.norm(v)
+    return np.linalg.norm(v) * (np.eye(3) - 3.0 * np.outer(u, u)) / 3.0
