@@ -353,6 +353,10 @@ def celio_bench(args):
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "results_checksum": float(np.sum(rows)),
     }
+    per_inst_bytes = measured_traffic("celio_mu4V_n8192", "k_celio_evolve")
+    if per_inst_bytes is not None:
+        line["roofline"]["traffic"] = per_inst_bytes * nori
+        line["roofline"]["traffic_source"] = "profiles/celio_mu4V_n8192_hbm_traffic.json"
     print(json.dumps(line), flush=True)
 
 
