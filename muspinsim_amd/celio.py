@@ -94,6 +94,52 @@ def h_contribs(Is, terms):
     return out
 
 
+class SystemSplit:
+    """The contributions of a system's terms, computed once, with the Zeeman
+    terms of a field merged in per configuration: the Zeeman term of spin i
+    comes after spin i's own terms, so it joins their last run when that run
+    is a linear term on spin i and otherwise follows it (the grouping of
+    celio.py:88-100 applied to the term list system + Zeeman)."""
+
+    def __init__(self, Is, terms):
+        self.Is = list(Is)
+        self.dims = [int(round(2 * I + 1)) for I in Is]
+        parts = [_term_parts(t) for t in terms]
+        self._n_terms = len([p for p in parts if len(p[0]) > 0])
+        self._per_spin = []
+        for i in range(len(Is)):
+            mine = [p for p in parts if len(p[0]) > 0 and p[0][0] == i]
+            runs = []
+            for idx, run in itertools.groupby(mine, key=lambda p: p[0]):
+                runs.append((idx, sum(_small_matrix(self.Is, idx, T) for _, T in run)))
+            self._per_spin.append(runs)
+        self._S = [spin_matrices(I) for I in self.Is]
+
+    @property
+    def has_terms(self):
+        return self._n_terms > 0
+
+    def contribs(self, zeeman=None):
+        """CelioHContrib list; zeeman: per-spin field vectors gamma_i B (MHz)
+        or None (no field)."""
+        out = []
+        for i, runs in enumerate(self._per_spin):
+            runs = list(runs)
+            if zeeman is not None:
+                Hz = sum(zeeman[i][a] * self._S[i][a] for a in range(3))
+                if runs and runs[-1][0] == (i,):
+                    runs[-1] = ((i,), runs[-1][1] + Hz)
+                else:
+                    runs.append(((i,), Hz))
+            for idx, H in runs:
+                spins = [idx[0]] if len(idx) == 2 and idx[0] == idx[1] else list(idx)
+                others = [q for q in range(len(self.Is)) if q not in spins]
+                order = spins + others
+                out.append(CelioHContrib(H, int(np.prod([self.dims[q] for q in others])), order,
+                                         [self.dims[q] for q in order]))
+        return out
+
+
 def validate(terms, times):
     """validation.py:48-71."""
     times = np.asarray(times, dtype=float)
@@ -134,6 +180,43 @@ def _n_involved(c):
     return max(q, 1)
 
 
+def _fuse(contribs):
+    """Runs of consecutive contributions whose spin sets nest, as
+    [(frame spins, [(H, spins)])]: applying the run's factors one after the
+    other equals applying their product on the frame (the largest set)."""
+    groups = []
+    for c in contribs:
+        sp = tuple(c.spin_order[:_n_involved(c)])
+        if groups:
+            frame, parts = groups[-1]
+            if set(sp) <= set(frame):
+                parts.append((c.matrix, sp))
+                continue
+            if set(frame) < set(sp):
+                groups[-1] = (sp, parts + [(c.matrix, sp)])
+                continue
+        groups.append((sp, [(c.matrix, sp)]))
+    return groups
+
+
+def _embedding(dims, frame, part):
+    """Nonzeros of (factor on `part`) (x) identity on the other frame spins,
+    in the frame's index order: E[ia, ib] = c[ra, rb]; also the part size."""
+    fd = [dims[q] for q in frame]
+    m = int(np.prod(fd))
+    digits = np.array(np.unravel_index(np.arange(m), fd)).T          # (m, len(frame))
+    pos = [frame.index(q) for q in part]
+    rest = [a for a in range(len(frame)) if a not in pos]
+    pd = [dims[q] for q in part]
+    sub = np.ravel_multi_index(tuple(digits[:, pos].T), pd) if pos else np.zeros(m, int)
+    key = (np.ravel_multi_index(tuple(digits[:, rest].T), [fd[a] for a in rest])
+           if rest else np.zeros(m, int))
+    A, B = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    keep = key[A] == key[B]
+    ia, ib = A[keep], B[keep]
+    return ia, ib, sub[ia], sub[ib], int(np.prod(pd))
+
+
 def _exp_factors(eng, Hs, signs, theta):
     """exp(sign i theta H) for a batch of Hermitian matrices, by msp_eigh:
     U diag(exp(sign i theta E)) U^H (device tensor)."""
@@ -159,6 +242,7 @@ class CelioJob:
         self._ntab = 0
         self._H = []            # (H, sign) to exponentiate, in mats order
         self._fixed = []        # (offset, matrix) stored as given
+        self._composites = []   # (offset, frame spins, [(part offset, part spins, m)])
         self._mat_key = {}
         self._nmat = 0
         self._desc = []
@@ -193,18 +277,34 @@ class CelioJob:
             self._nmat += H.size
         return self._mat_key[key]
 
+    def _composite(self, frame, parts):
+        """Offset of the product of consecutive factors (parts applied in
+        order) embedded on the spins `frame`."""
+        key = ("composite", tuple(frame), tuple((o, tuple(sp)) for o, sp, _ in parts))
+        if key not in self._mat_key:
+            m = int(np.prod([self.dims[q] for q in frame]))
+            self._mat_key[key] = self._nmat
+            self._composites.append((self._nmat, tuple(frame), parts))
+            self._nmat += m * m
+        return self._mat_key[key]
+
     def add_config(self, contribs, meas_matrix, meas_spin, dagger=False):
         """contribs: CelioHContrib list in Trotter order; the measured operator
         acts on spin meas_spin.  dagger: evolve with c_j^H (density mode)."""
         first = len(self._desc)
-        for c in contribs:
-            m = c.matrix.shape[0]
-            spins = c.spin_order[:_n_involved(c)]
+        sign = +1 if dagger else -1
+        for frame, parts in _fuse(contribs):
+            m = int(np.prod([self.dims[q] for q in frame]))
             if m > 64:
                 raise NotImplementedError("Celio factors act on at most 64 states (two spins "
                                           "with dimensions up to 8 x 8)")
-            fib, el = self._table(spins)
-            self._desc.append((m, self._matrix(c.matrix, +1 if dagger else -1), fib, el))
+            fib, el = self._table(frame)
+            if len(parts) == 1:
+                off = self._matrix(parts[0][0], sign)
+            else:
+                off = self._composite(frame, [(self._matrix(H, sign), sp, H.shape[0])
+                                              for H, sp in parts])
+            self._desc.append((m, off, fib, el))
         self._cfg_steps.append((first, len(self._desc) - first))
         fib, el = self._table([meas_spin])
         mm = np.asarray(meas_matrix, dtype=complex)
@@ -276,6 +376,7 @@ class CelioJob:
             idx = torch.as_tensor(np.array([off for off, _, _ in items])[:, None] +
                                   np.arange(m * m)[None, :], device=dev)
             mats[idx.reshape(-1)] = C.reshape(-1)
+        self._compose(mats, dev)
         tables = np.concatenate(self._tables).astype(np.int32)
         for (m, _, fib, el) in self._desc + self._meas:
             if fib + self.n // m > tables.size or el + m > tables.size:
@@ -297,6 +398,28 @@ class CelioJob:
                         if wsb else None)
         prep["wsb"] = int(wsb)
         return prep
+
+    def _compose(self, mats, dev):
+        """Products of the fused factors, batched by pattern (frame and part
+        spins): F = E_last ... E_1 with E_j the j-th factor on the frame."""
+        pats = {}
+        for off, frame, parts in self._composites:
+            key = (frame, tuple(tuple(sp) for _, sp, _ in parts))
+            pats.setdefault(key, []).append((off, [o for o, _, _ in parts]))
+        for (frame, pspins), items in pats.items():
+            m = int(np.prod([self.dims[q] for q in frame]))
+            F = None
+            for j, sp in enumerate(pspins):
+                ia, ib, ra, rb, mp = _embedding(self.dims, frame, sp)
+                src = torch.as_tensor(np.array([po[j] for _, po in items])[:, None] +
+                                      (ra * mp + rb)[None, :], device=dev)
+                E = torch.zeros((len(items), m * m), dtype=torch.complex128, device=dev)
+                E[:, torch.as_tensor(ia * m + ib, device=dev)] = mats[src]
+                E = E.reshape(len(items), m, m)
+                F = E if F is None else torch.matmul(E, F)
+            dst = torch.as_tensor(np.array([off for off, _ in items])[:, None] +
+                                  np.arange(m * m)[None, :], device=dev)
+            mats[dst.reshape(-1)] = F.reshape(-1)
 
     def launch(self, prep, ntimes, out=None):
         """One msp_celio_evolve over every state; (nstates, ntimes) device

@@ -326,15 +326,16 @@ class ExperimentRunner:
         avg = cfg.celio_averages
         dt = times[1] - times[0] if times.size > 1 else 0.0
         job = celio.CelioJob(s.Is, dt, cfg.celio_k)
+        split = celio.SystemSplit(s.Is, s.terms)
         Sm = spin_matrices(0.5)
         combine = []
         for snap in snaps:
             w = self.load_config(snap)
-            terms = list(s.terms)
-            if not np.array_equal(self._B, [0, 0, 0]):
-                terms += [_Term([i], self._B * g) for i, g in enumerate(s.gammas)]
-            celio.validate(terms, times)
-            contribs = celio.h_contribs(s.Is, terms)
+            field = not np.array_equal(self._B, [0, 0, 0])
+            if not field and not split.has_terms:
+                raise ValueError("No interaction terms to evolve")
+            celio.validate([_Term([0], np.zeros(3))], times)
+            contribs = split.contribs([self._B * g for g in s.gammas] if field else None)
             if avg > 0:
                 if self._T != np.inf:
                     raise ValueError("The fast version of Celio's method requires T -> inf. "

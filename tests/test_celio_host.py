@@ -114,6 +114,16 @@ def _emulate(job, ntimes):
         mats[off:off + M.size] = M.reshape(-1)
     for off, H, sign in job._H:
         mats[off:off + H.size] = expm(sign * 2j * np.pi * H * job.dt / job.k).reshape(-1)
+    for off, frame, parts in job._composites:
+        # fused run: product of the parts, each as a full-space factor
+        # restricted to the frame (built by the oracle's own embedding)
+        fd = [job.dims[q] for q in frame]
+        F = np.eye(int(np.prod(fd)), dtype=complex)
+        for po, sp, m in parts:
+            c = mats[po:po + m * m].reshape(m, m)
+            E = oc.full_operator(fd, tuple(frame.index(q) for q in sp), c)
+            F = E @ F
+        mats[off:off + F.size] = F.reshape(-1)
     tab = np.concatenate(job._tables)
     ivec = np.concatenate(job._init_vec)
     out = np.zeros((job.nstates, ntimes))
@@ -202,3 +212,38 @@ def test_job_muon_not_first_and_spin_order():
     got = np.asarray(wts) @ _emulate(job, t.size)
     ref = np.real(oc.evolve(Is, terms, 4, t, rho0, O))
     assert np.abs(got - ref).max() <= 1e-12
+
+
+def test_embedding_matches_full_operator():
+    """The index map that embeds a fused part on its frame equals the
+    oracle's (c on the part's spins) (x) identity, for parts in any order."""
+    from muspinsim_amd.celio import _embedding
+    rng = np.random.default_rng(2)
+    dims = [2, 3, 2, 4]
+    for frame, part in [((0, 1), (0,)), ((0, 1), (1,)), ((1, 0), (0,)), ((2, 3), (3, 2)),
+                        ((3, 1), (1, 3)), ((0, 1), (0, 1)), ((0, 1), (1, 0))]:
+        mp = int(np.prod([dims[q] for q in part]))
+        c = rng.normal(size=(mp, mp)) + 1j * rng.normal(size=(mp, mp))
+        ia, ib, ra, rb, m2 = _embedding(dims, list(frame), list(part))
+        fd = [dims[q] for q in frame]
+        m = int(np.prod(fd))
+        E = np.zeros((m, m), dtype=complex)
+        E[ia, ib] = c[ra, rb]
+        ref = oc.full_operator(fd, tuple(frame.index(q) for q in part), c)
+        assert m2 == mp and np.array_equal(E, ref)
+
+
+def test_system_split_matches_grouping_of_full_term_list():
+    """Zeeman terms merged per configuration give the same contributions as
+    grouping the term list system + Zeeman (celio.py:88-100)."""
+    from muspinsim_amd import celio
+    Is, terms = _random_system()
+    terms = terms + [("linear", 2, 2, [0.1, 0.2, 0.3])]           # spin 2 ends on a linear run
+    z = [np.array([0.3, -0.2, 0.5]) * (k + 1) for k in range(3)]
+    split = celio.SystemSplit(Is, _as_terms(terms))
+    got = split.contribs(z)
+    ref = celio.h_contribs(Is, _as_terms(terms + [("linear", i, i, z[i]) for i in range(3)]))
+    assert len(got) == len(ref)
+    for a, b in zip(got, ref):
+        assert a.spin_order == b.spin_order and np.allclose(a.matrix, b.matrix, atol=1e-14)
+    assert len(split.contribs(None)) == len(celio.h_contribs(Is, _as_terms(terms)))
