@@ -240,10 +240,12 @@ int msp_accumulate(int nslots, int width, const int32_t *d_ptr, const int32_t *d
  * row-major complex matrix at d_mats + 2*offset acts on the fibers whose
  * base indices are d_tables[fiber .. fiber + n/m) through the m element
  * offsets d_tables[element .. element + m).  d_meas[4c..4c+3] describes O_c
- * (m = 2, the muon).  init_mode 0: psi = mu (x) exp(2 pi i phi)/sqrt(n/2)
- * with the muon first, mu = the 2 complex at d_init_vec + 2*d_init_idx[s],
- * phi = d_phases[s*(n/2) ..]; init_mode 1: product state
- * psi[I] = prod_q v_q[digit_q(I)] with v_q at d_init_vec + 2*d_init_idx[s*n_spins + q].
+ * (an operator on one spin of dimension 2..10).  init_mode 0:
+ * psi = mu (x) exp(2 pi i phi)/sqrt(n/2) with the muon first, mu = the 2
+ * complex at d_init_vec + 2*d_init_idx[s], phi = d_phases[s*(n/2) ..];
+ * init_mode 1: product state psi[I] = prod_q v_q[digit_q(I)] with v_q at
+ * d_init_vec + 2*d_init_idx[s*n_spins + q]; init_mode 2: the n complex at
+ * d_init_vec + 2*d_init_idx[s].
  * n > 8192 keeps the states in d_work (msp_celio_workspace bytes); smaller
  * n keeps each state in LDS and needs no workspace.  Factor sizes m: any
  * m <= 64. */

@@ -308,8 +308,10 @@ class CelioJob:
         self._cfg_steps.append((first, len(self._desc) - first))
         fib, el = self._table([meas_spin])
         mm = np.asarray(meas_matrix, dtype=complex)
-        if mm.shape != (2, 2) or self.dims[meas_spin] != 2:
-            raise NotImplementedError("Celio evolution measures operators on a spin-1/2")
+        d = self.dims[meas_spin]
+        if mm.shape != (d, d) or not 2 <= d <= 10:
+            raise NotImplementedError("Celio evolution measures operators on one spin of "
+                                      "dimension 2 to 10")
         self._meas.append((mm.shape[0], self._matrix(mm), fib, el))
         return len(self._cfg_steps) - 1
 
@@ -351,6 +353,22 @@ class CelioJob:
             self._init_idx.append([c[0] for c in combo])
             weights.append(wt)
         return weights
+
+    def add_vector_states(self, cfg, vectors, weights):
+        """States given as the columns of `vectors` (n, r) with `weights`;
+        returns the weights kept (nonzero)."""
+        if self.mode not in (None, 2):
+            raise ValueError("one job holds one kind of initial state")
+        self.mode = 2
+        kept = []
+        V = np.asarray(vectors, dtype=complex)
+        for r in range(V.shape[1]):
+            if weights[r] == 0.0:
+                continue
+            self._state_cfg.append(cfg)
+            self._init_idx.append(self._vec(V[:, r]))
+            kept.append(float(weights[r]))
+        return kept
 
     @property
     def nstates(self):
@@ -459,6 +477,17 @@ class CelioJob:
         return self.launch(prep, ntimes).cpu().numpy()
 
 
+@dataclass
+class EvolveContrib:
+    """A Trotter factor as the native evolution takes it (cpp/celio.cpp:4-10):
+    the factor on its spins, the other spins' dimension, and the index map
+    of the state ordered with the factor's spins first."""
+
+    matrix: np.ndarray
+    other_dim: int
+    indices: np.ndarray
+
+
 def plus_state(sigma):
     """+1 eigenvector of a 2 x 2 Pauli combination (celio.py:380-383)."""
     w, v = np.linalg.eigh(np.asarray(sigma, dtype=complex))
@@ -527,17 +556,32 @@ class CelioHamiltonian:
         return h_contribs(self._spinsys.Is, self._terms)
 
     def _calc_trotter_evol_op_contribs(self, time_step, cpp=True):
-        """[(c_j (m x m ndarray), CelioHContrib)] with
-        c_j = exp(-2 pi i H_j dt / k) on the contribution's own spins
-        (celio.py:145-208), computed by msp_eigh."""
+        """Trotter factors c_j = exp(-2 pi i H_j dt / k) (celio.py:145-208),
+        computed by msp_eigh: with cpp, EvolveContrib objects (the m x m
+        matrix, the other spins' dimension, and the index map that orders the
+        state with the factor's spins first); otherwise each factor as an
+        n x n sparse matrix on the whole system."""
+        from scipy import sparse
         from . import gpu
         eng = gpu.engine()
-        out = []
+        dims = [int(round(2 * I + 1)) for I in self._spinsys.Is]
+        n = int(np.prod(dims))
         theta = 2.0 * np.pi * time_step / self._k
+        out = []
         for c in h_contribs(self._spinsys.Is, self._terms):
             Hs = torch.as_tensor(c.matrix[None], dtype=torch.complex128, device=eng.device)
             sg = torch.full((1,), -1.0, dtype=torch.float64, device=eng.device)
-            out.append((_exp_factors(eng, Hs, sg, theta)[0].cpu().numpy(), c))
+            C = _exp_factors(eng, Hs, sg, theta)[0].cpu().numpy()
+            idx = np.transpose(np.arange(n, dtype=np.uint64).reshape(dims),
+                               axes=c.spin_order).flatten()
+            if cpp:
+                out.append(EvolveContrib(C, int(c.other_dimension), idx))
+            else:
+                K = sparse.kron(sparse.csr_matrix(C), sparse.identity(c.other_dimension),
+                                format="coo")
+                out.append(sparse.csr_matrix((K.data, (idx[K.row].astype(np.int64),
+                                                       idx[K.col].astype(np.int64))),
+                                             shape=(n, n)))
         return out
 
     def evolve(self, rho0, times, operators):
@@ -562,11 +606,18 @@ class CelioHamiltonian:
                 raise NotImplementedError("Celio evolution measures operators on one spin")
             ops.append(form)
         if prod is None:
-            raise NotImplementedError("Celio evolution needs a product initial state")
+            # a general density matrix: its eigenvectors, weighted by the
+            # eigenvalues (msp_eigh)
+            from . import gpu
+            w, V = gpu.eigh(rho)
+            w = np.where(w > 1e-14 * max(w.max(), 1e-300), w, 0.0)
         weights = []
         for q, A in ops:
             cfg = job.add_config(contribs, A, q, dagger=True)
-            weights.append(job.add_product_states(cfg, prod))
+            if prod is not None:
+                weights.append(job.add_product_states(cfg, prod))
+            else:
+                weights.append(job.add_vector_states(cfg, V, w))
         out = job.run(times.size)
         res = np.zeros((times.size, len(ops)), dtype=complex)
         s0 = 0

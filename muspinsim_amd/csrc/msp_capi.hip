@@ -437,8 +437,8 @@ int msp_celio_evolve(int n, int n_spins, const int32_t *h_dims, int nstates, int
     if (nstates < 0 || ntimes < 0 || k < 1)
         return fail(MSP_ERR_VALUE, "msp_celio_evolve: bad nstates / ntimes / k");
     if (nstates == 0 || ntimes == 0) return MSP_OK;
-    if (init_mode != 0 && init_mode != 1)
-        return fail(MSP_ERR_VALUE, "msp_celio_evolve: init_mode must be 0 or 1");
+    if (init_mode < 0 || init_mode > 2)
+        return fail(MSP_ERR_VALUE, "msp_celio_evolve: init_mode must be 0, 1 or 2");
     if (init_mode == 0 && (h_dims[0] != 2 || !d_phases))
         return fail(MSP_ERR_VALUE, "msp_celio_evolve: random states need the muon first and phases");
     if (!d_state_cfg || !d_cfg_steps || !d_desc || !d_meas || !d_mats || !d_tables ||

@@ -39,7 +39,7 @@ __device__ inline double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// <x|O|x> over the fibers of the muon (M = 2), this thread's part
+// <x|O|x> for an operator on one spin (M = its dimension), this thread's part
 template <int M>
 __device__ __attribute__((always_inline)) double measure_part(const double2 *psi, int n, int4 md, const double2 *__restrict__ mats,
                                const int *__restrict__ tables)
@@ -178,6 +178,10 @@ __global__ __launch_bounds__(512) void k_celio_evolve(
             const double2 a = mu[j];
             psi[I] = make_double2((a.x * cs - a.y * sn) * sc, (a.x * sn + a.y * cs) * sc);
         }
+    } else if (init_mode == 2) {
+        // explicit state vector
+        const double2 *v = init_vec + init_idx[s];
+        for (int I = threadIdx.x; I < n; I += blockDim.x) psi[I] = v[I];
     } else {
         // product state: psi[I] = prod_q v_q[digit_q(I)], spin 0 slowest
         const int *idx = init_idx + (size_t)s * dims.n_spins;
@@ -198,7 +202,19 @@ __global__ __launch_bounds__(512) void k_celio_evolve(
     const int first = cfg_steps[2 * cfg], count = cfg_steps[2 * cfg + 1];
     const int4 md = meas[cfg];
     for (int t = 0; t < ntimes; ++t) {
-        const double v = block_sum(measure_part<2>(psi, n, md, mats, tables), red);
+        double part;
+        switch (md.x) {
+        case 3: part = measure_part<3>(psi, n, md, mats, tables); break;
+        case 4: part = measure_part<4>(psi, n, md, mats, tables); break;
+        case 5: part = measure_part<5>(psi, n, md, mats, tables); break;
+        case 6: part = measure_part<6>(psi, n, md, mats, tables); break;
+        case 7: part = measure_part<7>(psi, n, md, mats, tables); break;
+        case 8: part = measure_part<8>(psi, n, md, mats, tables); break;
+        case 9: part = measure_part<9>(psi, n, md, mats, tables); break;
+        case 10: part = measure_part<10>(psi, n, md, mats, tables); break;
+        default: part = measure_part<2>(psi, n, md, mats, tables); break;
+        }
+        const double v = block_sum(part, red);
         if (threadIdx.x == 0) out[(size_t)s * ntimes + t] = v;
         if (t == ntimes - 1) break;
         for (int r = 0; r < k; ++r)

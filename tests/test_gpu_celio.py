@@ -39,14 +39,24 @@ def test_trotter_factors_known_answers(gpu_lib):
     ssys = SpinSystem(["mu", "e"], celio_k=10)
     ssys.add_linear_term(0, [1, 0, 1])
     ssys.add_bilinear_term(0, 1, np.eye(3))
-    f = ssys.hamiltonian._calc_trotter_evol_op_contribs(1)
+    f = ssys.hamiltonian._calc_trotter_evol_op_contribs(1, True)
     assert len(f) == 2
-    assert np.allclose(f[0][0], [[0.902916796 - 0.30392537j, -0.30392537j],
-                                 [-0.30392537j, 0.902916796 + 0.30392537j]], atol=1e-8)
-    E = oc.full_operator([2, 2], (0,), f[0][0]) @ oc.full_operator([2, 2], (0, 1), f[1][0])
-    assert np.allclose(E[0], [0.84425599 - 0.44143085j, -0.09276182 - 0.01469203j,
-                              0.04521741 - 0.28549151j, 0], atol=1e-8)
-    assert f[0][1].other_dimension == 2 and f[0][1].spin_order == [0, 1]
+    assert np.allclose(f[0].matrix, [[0.902916796 - 0.30392537j, -0.30392537j],
+                                     [-0.30392537j, 0.902916796 + 0.30392537j]], atol=1e-8)
+    assert f[0].other_dim == 2 and np.array_equal(f[0].indices, [0, 1, 2, 3])
+    full = ssys.hamiltonian._calc_trotter_evol_op_contribs(1, False)
+    E = (full[0] @ full[1]).toarray()
+    ref = [[0.84425599 - 0.44143085j, -0.09276182 - 0.01469203j, 0.04521741 - 0.28549151j, 0],
+           [0, 0.89336999 - 0.15115734j, -0.049114 - 0.29027351j, -0.0475444 - 0.30018354j],
+           [-0.0475444 - 0.30018354j, 0.13640963 - 0.26088945j, 0.80293516 + 0.41982569j, 0],
+           [0, 0.04521741 - 0.28549151j, -0.09276182 - 0.01469203j, 0.93934479 + 0.15893624j]]
+    assert np.allclose(E, ref, atol=1e-8)
+    e = SpinSystem(["mu", "e"], celio_k=10)
+    e.add_linear_term(0, [1, 0, 0])
+    c = e.hamiltonian._calc_trotter_evol_op_contribs(1, False)
+    cs, sn = 0.95105652, -0.30901699j
+    assert np.allclose(c[0].toarray(), [[cs, 0, sn, 0], [0, cs, 0, sn], [sn, 0, cs, 0],
+                                        [0, sn, 0, cs]])
 
 
 @pytest.mark.gpu
@@ -115,6 +125,25 @@ def _run_text(text):
     from muspinsim_amd.input import MuSpinInput
     er = ExperimentRunner(MuSpinInput(StringIO(text)))
     return er, er.run()
+
+
+@pytest.mark.gpu
+def test_evolve_general_state_and_spin1_operator(gpu_lib):
+    """A density matrix that is not a product of one-spin states (through
+    its eigenvectors) and an operator on the spin-1 nucleus."""
+    s = _system()
+    t = np.linspace(0, 0.3, 12)
+    rng = np.random.default_rng(8)
+    G = rng.normal(size=(12, 12)) + 1j * rng.normal(size=(12, 12))
+    rho = G @ G.conj().T
+    rho /= np.trace(rho)
+    from muspinsim_amd.spinop import DensityOperator
+    rho0 = DensityOperator(rho, dim=(2, 3, 2))
+    op = s.operator({1: "z"})
+    got = s.hamiltonian.evolve(rho0, t, [op])[:, 0]
+    ref = oc.evolve(s.Is, _oracle_terms(s), 7, t, rho,
+                    op.matrix.toarray() if hasattr(op.matrix, "toarray") else op.matrix)
+    assert np.abs(got - ref).max() <= PARITY
 
 
 @pytest.mark.gpu
