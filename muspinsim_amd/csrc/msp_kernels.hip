@@ -349,13 +349,45 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
             Vs[r * LDV + l] = (l < nbk) ? vblock(n, A, k0, r, l) : cmk(0.0, 0.0);
         }
         __syncthreads();
-        MSP_PFOR(idx, nbk * nbk)
+        // Gram G(i, j) = V(:, i)^H V(:, j), i < j: each of the 120 pairs is
+        // split over two threads (rows halved, four accumulators each); the
+        // second half's partial goes through T, which is rebuilt below
         {
-            const int i = idx / nbk, j = idx % nbk;
-            cdbl acc = cmk(0.0, 0.0);
-            if (i < j)
-                for (int r = k0 + j + 1; r < n; ++r) cfmac(acc, Vs[r * LDV + i], Vs[r * LDV + j]);
-            G[i * MSP_BT_NB + j] = acc;
+            const int t = threadIdx.x;
+            const int q = t & 127, h = t >> 7;
+            int i = 0, rem = q;
+            while (i < MSP_BT_NB - 1 && rem >= MSP_BT_NB - 1 - i) {
+                rem -= MSP_BT_NB - 1 - i;
+                ++i;
+            }
+            const int j = i + 1 + rem;
+            cdbl part = cmk(0.0, 0.0);
+            if (q < MSP_BT_NB * (MSP_BT_NB - 1) / 2 && j < nbk) {
+                const int r0 = k0 + j + 1;
+                const int mid = r0 + (n - r0) / 2;
+                const int ra = h ? mid : r0, rb = h ? n : mid;
+                cdbl a0 = cmk(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+                int r = ra;
+                for (; r + 3 < rb; r += 4) {
+                    cfmac(a0, Vs[r * LDV + i], Vs[r * LDV + j]);
+                    cfmac(a1, Vs[(r + 1) * LDV + i], Vs[(r + 1) * LDV + j]);
+                    cfmac(a2, Vs[(r + 2) * LDV + i], Vs[(r + 2) * LDV + j]);
+                    cfmac(a3, Vs[(r + 3) * LDV + i], Vs[(r + 3) * LDV + j]);
+                }
+                for (; r < rb; ++r) cfmac(a0, Vs[r * LDV + i], Vs[r * LDV + j]);
+                part = cmk(a0.re + a1.re + a2.re + a3.re, a0.im + a1.im + a2.im + a3.im);
+            }
+            if (t < 2 * 128 && q < MSP_BT_NB * (MSP_BT_NB - 1) / 2)
+                (h ? T : G)[i * MSP_BT_NB + j] = part;
+        }
+        __syncthreads();
+        {
+            const int t = threadIdx.x;
+            const int i = t / MSP_BT_NB, j = t % MSP_BT_NB;
+            if (t < MSP_BT_NB * MSP_BT_NB && i < j) {
+                const cdbl a = G[t], b = T[t];
+                G[t] = cmk(a.re + b.re, a.im + b.im);
+            }
         }
         __syncthreads();
         // T = (diag(1/tau) + strictly upper part of V^H V)^{-1}, by back
