@@ -324,6 +324,8 @@ __global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, i
 
 size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 20 * sizeof(int)) + 16; }
 
+constexpr int kLarftStageMaxDev = 256;   // n up to which k_larft stages the block in LDS
+
 // T factors of every reflector block, and V T for the MFMA back-transformation:
 // grid (blocks, instances).  With STAGE the block's reflectors are first
 // copied to LDS (n x 16, rows padded to 17), so that the Gram matrix and V T
@@ -343,10 +345,20 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
     const cdbl *A = (const cdbl *)H + (size_t)b * n * n;
     cdbl *Vs = (cdbl *)smem;
     if constexpr (STAGE) {
-        MSP_PFOR(idx, n * MSP_BT_NB)
-        {
+        // all of a thread's loads are issued before any LDS store (n <= 256,
+        // so at most 16 elements per thread)
+        constexpr int PER = (kLarftStageMaxDev * MSP_BT_NB + 255) / 256;
+        cdbl tmp[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int idx = threadIdx.x + 256 * q;
             const int r = idx / MSP_BT_NB, l = idx % MSP_BT_NB;
-            Vs[r * LDV + l] = (l < nbk) ? vblock(n, A, k0, r, l) : cmk(0.0, 0.0);
+            tmp[q] = (idx < n * MSP_BT_NB && l < nbk) ? vblock(n, A, k0, r, l) : cmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            if (idx < n * MSP_BT_NB) Vs[(idx / MSP_BT_NB) * LDV + idx % MSP_BT_NB] = tmp[q];
         }
         __syncthreads();
         // Gram G(i, j) = V(:, i)^H V(:, j), i < j: each of the 120 pairs is
@@ -394,8 +406,12 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
         // substitution one column per thread, multiplied through by tau_i so
         // that tau_i = 0 (no reflection) gives a zero row:
         //   T(j, j) = tau_j,  T(i, j) = -tau_i sum_{k = i+1..j} G(i, k) T(k, j)
+        // (each thread's column is kept in registers: static indices after
+        // unrolling, terms with k > j are zero since tc[k] = 0 there)
         if (threadIdx.x < MSP_BT_NB) {
             const int j = threadIdx.x;
+            cdbl tc[MSP_BT_NB];
+#pragma unroll
             for (int i = MSP_BT_NB - 1; i >= 0; --i) {
                 cdbl t = cmk(0.0, 0.0);
                 if (j < nbk && i <= j) {
@@ -403,13 +419,16 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
                         t = tau[k0 + j];
                     } else {
                         cdbl acc = cmk(0.0, 0.0);
-                        for (int k = i + 1; k <= j; ++k) cfma(acc, G[i * MSP_BT_NB + k], T[k * MSP_BT_NB + j]);
+#pragma unroll
+                        for (int k = i + 1; k < MSP_BT_NB; ++k) cfma(acc, G[i * MSP_BT_NB + k], tc[k]);
                         const cdbl ti = tau[k0 + i];
                         t = cmul(cmk(-ti.re, -ti.im), acc);
                     }
                 }
-                T[i * MSP_BT_NB + j] = t;
+                tc[i] = t;
             }
+#pragma unroll
+            for (int i = 0; i < MSP_BT_NB; ++i) T[i * MSP_BT_NB + j] = tc[i];
         }
         __syncthreads();
     } else {
@@ -451,7 +470,7 @@ __global__ void __launch_bounds__(256) k_larft(int n, const double *H, double *w
 }
 
 // LDS-staged WY factors up to this order (n x 17 complex, 52 KB at n = 192)
-constexpr int kLarftStageMax = 256;
+constexpr int kLarftStageMax = kLarftStageMaxDev;
 
 hipError_t launch_larft(int n, int batch, const double *H, double *work, hipStream_t s)
 {
