@@ -509,3 +509,52 @@ def test_input_file_closed_forms_cpu(name):
     systems, batch = er.instance_batch()
     t = np.asarray(er.config.x_axis_values)
     assert np.abs(opipe.run(systems, batch)[0] - CLOSED_FORMS[name](t)).max() <= 1e-12
+
+
+# ------------------------------------------------------------------ utils surface
+def test_utils_known_answers():
+    """The reference's tests/test_utils.py known answers."""
+    import copy
+    from muspinsim_amd.utils import (Clonable, Quaternion, deepmap, eulrange_gen, get_xy,
+                                     quat_from_polar, zcw_gen)
+
+    class TC(Clonable):
+        def __init__(self):
+            self.data = {"x": [1, 2, 3]}
+    tc = TC()
+    tc2 = tc.clone()
+    tc2.data["x"][0] = 2
+    assert tc.data["x"][0] == 1
+    assert deepmap(lambda x: x ** 2, [[1, 2, 3], [4, 5], [6, [7, 8]]]) == \
+        [[1, 4, 9], [16, 25], [36, [49, 64]]]
+    z1 = quat_from_polar(np.pi / 4.0, 0).rotate([0, 0, 1])
+    assert np.allclose(z1, [2 ** -0.5, 0, 2 ** -0.5])
+    th, ph = 0.6 * np.pi, 0.4 * np.pi
+    q2 = quat_from_polar(th, ph)
+    st, ct, sp, cp = np.sin(th), np.cos(th), np.sin(ph), np.cos(ph)
+    assert np.allclose(q2.rotate([0, 0, 1]), [st * cp, st * sp, ct])
+    assert np.allclose(q2.conjugate().rotate([0, 0, 1]), [-st * cp, st * sp, ct])
+    f = 3.0 * np.cos(zcw_gen(1000)[:, 0]) ** 2 - 1.0
+    assert abs(np.average(f)) < 1e-5
+    ow = eulrange_gen(10)
+    A = np.array([[1, 2, 3], [2, 4, 4], [3, 4, -3]])
+    Asum = sum(Quaternion.from_euler_angles(a, b, c).rotation_matrix() @ A @
+               Quaternion.from_euler_angles(a, b, c).rotation_matrix().T * w
+               for a, b, c, w in ow) / np.sum(ow[:, -1])
+    assert np.sum(np.abs(Asum - np.eye(3) * np.trace(A) / 3)) < 1.0
+    x, y = get_xy(np.array([0, 0, 1]))
+    assert np.allclose(x, [1, 0, 0]) and np.allclose(y, [0, 1, 0])
+    x, y = get_xy(np.array([1.0, 1.0, 0.5]))
+    assert np.allclose(np.cross(x, y), np.array([1.0, 1.0, 0.5]) / 1.5)
+    q = Quaternion.from_euler_angles(0.3, 1.1, -0.4)
+    assert np.allclose(Quaternion.from_euler_angles(*q.euler_angles()).rotation_matrix(),
+                       q.rotation_matrix())
+
+
+def test_package_surface():
+    import muspinsim_amd as m
+    from muspinsim_amd.simconfig import MuSpinConfig as SC
+    assert m.MuSpinConfig is SC and m.MuSpinInput.__name__ == "MuSpinInput"
+    assert m.ExperimentRunner.__name__ == "ExperimentRunner"
+    assert m.FittingRunner.__name__ == "FittingRunner" and m.constants.MU_TAU == 2.19703
+    assert isinstance(m.__version__, str)
