@@ -131,6 +131,22 @@ int msp_apply_kron(int n, int batch, int n_spins, const int32_t *h_dims, const d
 int msp_gemm_cn(int n, int batch, const double *d_A, const double *d_B, double *d_C,
                 void *stream);
 
+/* C[b] = A[b]^H B[b] on the 64 x 64 blocks on and above the block diagonal
+ * only (the others are left unwritten): the basis change of a Hermitian
+ * observable, O = U^H (M U), for consumers that read the upper blocks
+ * (Operator.basis_change, spinop.py:330-355, as msp_gemm_cn). */
+int msp_gemm_cn_upper(int n, int batch, const double *d_A, const double *d_B, double *d_C,
+                      void *stream);
+
+/* out[b] = w[b] * out_scale * Re sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)) with
+ * C_ij = alpha |O_ij|^2 + beta delta_ij Re O_ii, read from the upper blocks of a
+ * Hermitian O as msp_gemm_cn_upper writes them (the blocks above the diagonal
+ * count twice).  The fused form of msp_form_c + msp_integral for a real C
+ * (Hamiltonian.integrate_decaying, hamiltonian.py:150-162). */
+int msp_integral_obs(int n, int batch, const double *d_E, const double *d_O, double alpha,
+                     double beta, double tau, double out_scale, const double *d_w, double *d_out,
+                     void *stream);
+
 /* Coefficient matrix of the time sum.
  * real_mode = 0: C_ij = rho'_ij O'_ji (complex, d_rho required).
  * real_mode = 1: C_ij = alpha |O'_ij|^2 + beta delta_ij Re O'_ii (real). */

@@ -208,6 +208,31 @@ int msp_gemm_cn(int n, int batch, const double *d_A, const double *d_B, double *
     return hip_status(launch_gemm_cn(n, batch, d_A, d_B, d_C, S(stream)), "msp_gemm_cn");
 }
 
+int msp_gemm_cn_upper(int n, int batch, const double *d_A, const double *d_B, double *d_C,
+                      void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_gemm_cn_upper: n out of range");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_gemm_cn_upper: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_A || !d_B || !d_C) return fail(MSP_ERR_VALUE, "msp_gemm_cn_upper: null pointer");
+    return hip_status(launch_gemm_cn_upper(n, batch, d_A, d_B, d_C, S(stream)),
+                      "msp_gemm_cn_upper");
+}
+
+int msp_integral_obs(int n, int batch, const double *d_E, const double *d_O, double alpha,
+                     double beta, double tau, double out_scale, const double *d_w, double *d_out,
+                     void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_integral_obs: n out of range");
+    if (!(tau > 0.0)) return fail(MSP_ERR_VALUE, "msp_integral_obs: tau must be > 0");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_integral_obs: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_E || !d_O || !d_out) return fail(MSP_ERR_VALUE, "msp_integral_obs: null pointer");
+    return hip_status(launch_integral_obs(n, batch, d_E, d_O, alpha, beta, tau, out_scale, d_w,
+                                          d_out, S(stream)),
+                      "msp_integral_obs");
+}
+
 int msp_form_c(int n, int batch, const double *d_rho, const double *d_obs, int real_mode,
                double alpha, double beta, double *d_C, void *stream)
 {

@@ -13,9 +13,13 @@
 
 namespace {
 
+// UPPER: only the 64 x 64 blocks on or above the block diagonal (for a
+// Hermitian product whose consumer reads those blocks only).
+template <bool UPPER>
 __global__ void __launch_bounds__(256) k_gemm_cn_mfma(int n, const double *A, const double *B,
                                                       double *C)
 {
+    if (UPPER && blockIdx.y > blockIdx.x) return;
     const int b = blockIdx.z;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
@@ -78,9 +82,12 @@ __global__ void __launch_bounds__(256) k_gemm_cn_mfma(int n, const double *A, co
 }  // namespace
 
 hipError_t launch_gemm_cn_mfma(int n, int batch, const double *A, const double *B, double *C,
-                               hipStream_t s)
+                               hipStream_t s, bool upper)
 {
     const unsigned t = (unsigned)((n + 63) / 64);
-    hipLaunchKernelGGL(k_gemm_cn_mfma, dim3(t, t, batch), dim3(256), 0, s, n, A, B, C);
+    if (upper)
+        hipLaunchKernelGGL(k_gemm_cn_mfma<true>, dim3(t, t, batch), dim3(256), 0, s, n, A, B, C);
+    else
+        hipLaunchKernelGGL(k_gemm_cn_mfma<false>, dim3(t, t, batch), dim3(256), 0, s, n, A, B, C);
     return hipGetLastError();
 }

@@ -95,7 +95,7 @@ hipError_t launch_trace_products(int n, int nparam, const double *G, const doubl
 
 // Batched complex C = A^H B on FP64 matrix cores (msp_gemm_mfma.hip)
 hipError_t launch_gemm_cn_mfma(int n, int batch, const double *A, const double *B, double *C,
-                               hipStream_t s);
+                               hipStream_t s, bool upper = false);
 
 // Celio/Trotter evolution (msp_celio.hip)
 int celio_block_threads(int n);
@@ -106,3 +106,11 @@ hipError_t launch_celio_evolve(int n, int n_spins, const int32_t *dims, int nsta
                                const int32_t *tables, int init_mode, const double *init_vec,
                                const int32_t *init_idx, const double *phases, double *out,
                                double *work, hipStream_t s);
+
+// Hermitian products on the upper block triangle and the decaying integral
+// read from them (msp_kernels.hip)
+hipError_t launch_gemm_cn_upper(int n, int batch, const double *A, const double *B, double *C,
+                                hipStream_t s);
+hipError_t launch_integral_obs(int n, int batch, const double *E, const double *O, double alpha,
+                               double beta, double tau, double out_scale, const double *w,
+                               double *out, hipStream_t s);

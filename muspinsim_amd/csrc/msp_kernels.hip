@@ -758,6 +758,39 @@ __global__ void __launch_bounds__(256) k_integral(int n, const double *E, const 
     }
 }
 
+// Integral of C_ij = alpha |O_ij|^2 + beta delta_ij Re O_ii against
+// 1 / (1/tau + 2 pi i (E_i - E_j)), real part, read straight from O's 64 x 64
+// blocks on and above the block diagonal: O is Hermitian, so |O_ij| = |O_ji|,
+// and the real part of the kernel is even in E_i - E_j, so a block above the
+// diagonal counts twice and the blocks below are never read.
+__global__ void __launch_bounds__(256) k_integral_obs(int n, const double *E, const double *O,
+                                                      double alpha, double beta, double tau,
+                                                      double out_scale, const double *w,
+                                                      double *out)
+{
+    __shared__ double red[16];
+    const int b = blockIdx.x;
+    const double *Eb = E + (size_t)b * n;
+    const cdbl *Ob = (const cdbl *)O + (size_t)b * n * n;
+    const double a = 1.0 / tau;
+    const double twopi = 6.283185307179586;
+    double sre = 0.0;
+    MSP_PFOR(idx, n * n)
+    {
+        const int i = idx / n, j = idx % n;
+        const int bi = i >> 6, bj = j >> 6;
+        if (bi <= bj) {
+            const cdbl o = Ob[idx];
+            double c = alpha * (o.re * o.re + o.im * o.im);
+            if (i == j) c += beta * o.re;
+            const double bb = twopi * (Eb[i] - Eb[j]);
+            sre += ((bi < bj) ? 2.0 : 1.0) * c * a / (a * a + bb * bb);
+        }
+    }
+    sre = team_sum(sre, red);
+    if (threadIdx.x == 0) out[b] = (w ? w[b] : 1.0) * sre * out_scale;
+}
+
 // ---------------------------------------------------------------------------
 __global__ void k_accumulate(int nslots, int width, const int32_t *ptr, const int32_t *idx,
                              const double *rows, double scale, double *results)
@@ -1052,6 +1085,23 @@ hipError_t launch_gemm_cn(int n, int batch, const double *A, const double *B, do
 {
     ProfScope ps_("k_gemm_cn", s);
     return launch_gemm_cn_mfma(n, batch, A, B, C, s);
+}
+
+hipError_t launch_gemm_cn_upper(int n, int batch, const double *A, const double *B, double *C,
+                                hipStream_t s)
+{
+    ProfScope ps_("k_gemm_cn", s);
+    return launch_gemm_cn_mfma(n, batch, A, B, C, s, true);
+}
+
+hipError_t launch_integral_obs(int n, int batch, const double *E, const double *O, double alpha,
+                               double beta, double tau, double out_scale, const double *w,
+                               double *out, hipStream_t s)
+{
+    ProfScope ps_("k_integral", s);
+    hipLaunchKernelGGL(k_integral_obs, dim3(batch), dim3(256), 0, s, n, E, O, alpha, beta, tau,
+                       out_scale, w, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_form_c(int n, int batch, const double *rho, const double *obs, int real_mode,

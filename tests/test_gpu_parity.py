@@ -158,6 +158,47 @@ def test_gemm_cn(eng):
         assert np.abs(C - ref).max() <= 1e-13 * n
 
 
+@pytest.mark.parametrize("n", [16, 100, 192])
+def test_upper_block_gemm_and_observable_integral(eng, n):
+    """msp_gemm_cn_upper writes the blocks on and above the 64 x 64 block
+    diagonal bit-identically to msp_gemm_cn and leaves the others untouched;
+    msp_integral_obs on those blocks equals form_c + integral on the full
+    Hermitian O = U^H (M U) (numpy reference of the integral too)."""
+    from muspinsim_amd import _native as N
+    rng = np.random.default_rng(n)
+    nb = 3
+    _, U = np.linalg.eigh(np.stack([_herm(rng, n) for _ in range(nb)]))
+    M = np.stack([_herm(rng, n) for _ in range(nb)])
+    Y = M @ U
+    E = np.sort(rng.normal(size=(nb, n)) * 30, axis=1)
+    w = np.array([0.5, 1.0, 2.0])
+    dU, dY = _dev(U), _dev(Y)
+    full = eng.gemm_cn(dU, dY)
+    up = torch.full_like(full, 7.0)
+    st = eng.stream()
+    N.check(eng.lib.msp_gemm_cn_upper(n, nb, dU.data_ptr(), dY.data_ptr(), up.data_ptr(), st))
+    f, u = full.cpu().numpy(), up.cpu().numpy()
+    blk = np.arange(n) // 64
+    mask = blk[:, None] <= blk[None, :]
+    assert np.array_equal(u[:, mask], f[:, mask])
+    assert np.all(u[:, ~mask] == 7.0)
+    dE, dw = _dev(E, torch.float64), _dev(w, torch.float64)   # kept alive across the calls
+    for alpha, beta in ((0.25, 0.0), (2.0 / n, 1.0 / n)):
+        C = eng.form_c(None, full, True, alpha, beta)
+        ref = eng.integral(dE, C, True, 2.197, 1 / 2.197, dw).cpu().numpy()
+        out = torch.empty((nb,), dtype=torch.float64, device="cuda")
+        N.check(eng.lib.msp_integral_obs(n, nb, dE.data_ptr(), up.data_ptr(), alpha, beta, 2.197,
+                                         1 / 2.197, dw.data_ptr(), out.data_ptr(), st))
+        got = out.cpu().numpy()
+        for b in range(nb):
+            c = alpha * np.abs(f[b]) ** 2 + beta * np.diag(np.diag(f[b]).real)
+            npref = w[b] / 2.197 * np.real(np.sum(c / (1 / 2.197 + 2j * np.pi *
+                                                        (E[b][:, None] - E[b][None, :]))))
+            scale = np.abs(c).sum()
+            assert abs(got[b] - ref[b]) <= 1e-13 * scale
+            assert abs(got[b] - npref) <= 1e-12 * scale
+
+
 def test_timesum_and_integral(eng):
     rng = np.random.default_rng(3)
     for n, T in ((4, 7), (16, 130), (70, 65)):
