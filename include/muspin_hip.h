@@ -138,6 +138,15 @@ int msp_gemm_cn(int n, int batch, const double *d_A, const double *d_B, double *
 int msp_gemm_cn_upper(int n, int batch, const double *d_A, const double *d_B, double *d_C,
                       void *stream);
 
+/* O[b] = U[b]^H (f v_b.S_mu (x) 1) U[b] on the blocks on and above the block
+ * diagonal (as msp_gemm_cn_upper), with the muon operator applied while the
+ * second operand is loaded: the product msp_apply_muon_op then msp_gemm_cn
+ * form (experiment.py:486-496 via Operator.basis_change, spinop.py:330-355)
+ * in one pass for a spin-1/2 muon; f = 2 (Pauli) when pauli != 0. */
+int msp_gemm_muon_upper(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
+                        const double *d_vec, int pauli, const double *d_U, double *d_O,
+                        void *stream);
+
 /* out[b] = w[b] * out_scale * Re sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)) with
  * C_ij = alpha |O_ij|^2 + beta delta_ij Re O_ii, read from the upper blocks of a
  * Hermitian O as msp_gemm_cn_upper writes them (the blocks above the diagonal

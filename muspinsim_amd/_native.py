@@ -73,6 +73,7 @@ SIGNATURES = {
     "msp_trace_products": (_I, [_I, _I, _P, _P, _P, _P]),
     "msp_accumulate": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
     "msp_gemm_cn_upper": (_I, [_I, _I, _P, _P, _P, _P]),
+    "msp_gemm_muon_upper": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "msp_integral_obs": (_I, [_I, _I, _P, _P, _D, _D, _D, _D, _P, _P, _P]),
     "msp_celio_workspace": (_SZ, [_I, _I]),
     "msp_celio_evolve": (_I, [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P,

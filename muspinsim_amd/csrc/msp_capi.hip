@@ -219,6 +219,20 @@ int msp_gemm_cn_upper(int n, int batch, const double *d_A, const double *d_B, do
                       "msp_gemm_cn_upper");
 }
 
+int msp_gemm_muon_upper(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
+                        const double *d_vec, int pauli, const double *d_U, double *d_O,
+                        void *stream)
+{
+    if (n < 2 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_gemm_muon_upper: n out of range");
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_gemm_muon_upper: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_systems || !d_vec || !d_U || !d_O)
+        return fail(MSP_ERR_VALUE, "msp_gemm_muon_upper: null pointer");
+    return hip_status(launch_gemm_muon_upper(n, batch, d_systems, d_sys_id, d_vec, pauli, d_U, d_O,
+                                             S(stream)),
+                      "msp_gemm_muon_upper");
+}
+
 int msp_integral_obs(int n, int batch, const double *d_E, const double *d_O, double alpha,
                      double beta, double tau, double out_scale, const double *d_w, double *d_out,
                      void *stream)

@@ -7,6 +7,7 @@
 // per operand; complex products use four real MFMAs (msp_mfma.h).
 #include <hip/hip_runtime.h>
 
+#include "../../include/muspin_hip.h"
 #include "msp_kernels.h"
 #include "msp_mfma.h"
 #include "msp_team.h"
@@ -15,12 +16,30 @@ namespace {
 
 // UPPER: only the 64 x 64 blocks on or above the block diagonal (for a
 // Hermitian product whose consumer reads those blocks only).
-template <bool UPPER>
+// MIX: B = (f v.S_mu (x) 1) A formed while loading, for a spin-1/2 muon at
+// the stride its system gives: B[k] = M[a][a] A[k] + M[a][1-a] A[k'] with a
+// the muon digit of row k and k' the row with the other digit
+// (the product k_apply_muon writes, f = 2 for Pauli matrices).
+template <bool UPPER, bool MIX>
 __global__ void __launch_bounds__(256) k_gemm_cn_mfma(int n, const double *A, const double *B,
-                                                      double *C)
+                                                      double *C, const MspSystem *systems,
+                                                      const int32_t *sys_id, const double *vec,
+                                                      int pauli)
 {
     if (UPPER && blockIdx.y > blockIdx.x) return;
     const int b = blockIdx.z;
+    int mstride = 1;
+    cdbl m_same[2], m_other[2];
+    if constexpr (MIX) {
+        const MspSystem &S = systems[sys_id ? sys_id[b] : 0];
+        for (int k = S.n_spins - 1; k > S.muon_index; --k) mstride *= S.dims[k];
+        const double f = pauli ? 1.0 : 0.5;        // v.sigma or v.S
+        const double vx = vec[3 * b], vy = vec[3 * b + 1], vz = vec[3 * b + 2];
+        m_same[0] = cmk(f * vz, 0.0);              // M[0][0]
+        m_same[1] = cmk(-f * vz, 0.0);             // M[1][1]
+        m_other[0] = cmk(f * vx, -f * vy);         // M[0][1]
+        m_other[1] = cmk(f * vx, f * vy);          // M[1][0]
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
     const int i0 = blockIdx.y * 64 + 32 * (wave >> 1);
@@ -46,8 +65,17 @@ __global__ void __launch_bounds__(256) k_gemm_cn_mfma(int n, const double *A, co
             const int kc = (k < n) ? k : n - 1;
             av[h][0] = Ab[(size_t)kc * n + iac];
             av[h][1] = Ab[(size_t)kc * n + ibc];
-            bv[h][0] = Bb[(size_t)kc * n + jac];
-            bv[h][1] = Bb[(size_t)kc * n + jbc];
+            if constexpr (MIX) {
+                const int a = (kc / mstride) & 1;
+                const int k2 = kc + (a ? -mstride : mstride);
+                const cdbl x0 = Bb[(size_t)kc * n + jac], x1 = Bb[(size_t)kc * n + jbc];
+                const cdbl y0 = Bb[(size_t)k2 * n + jac], y1 = Bb[(size_t)k2 * n + jbc];
+                bv[h][0] = cadd(cmul(m_same[a], x0), cmul(m_other[a], y0));
+                bv[h][1] = cadd(cmul(m_same[a], x1), cmul(m_other[a], y1));
+            } else {
+                bv[h][0] = Bb[(size_t)kc * n + jac];
+                bv[h][1] = Bb[(size_t)kc * n + jbc];
+            }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -86,8 +114,21 @@ hipError_t launch_gemm_cn_mfma(int n, int batch, const double *A, const double *
 {
     const unsigned t = (unsigned)((n + 63) / 64);
     if (upper)
-        hipLaunchKernelGGL(k_gemm_cn_mfma<true>, dim3(t, t, batch), dim3(256), 0, s, n, A, B, C);
+        hipLaunchKernelGGL((k_gemm_cn_mfma<true, false>), dim3(t, t, batch), dim3(256), 0, s, n, A,
+                           B, C, nullptr, nullptr, nullptr, 0);
     else
-        hipLaunchKernelGGL(k_gemm_cn_mfma<false>, dim3(t, t, batch), dim3(256), 0, s, n, A, B, C);
+        hipLaunchKernelGGL((k_gemm_cn_mfma<false, false>), dim3(t, t, batch), dim3(256), 0, s, n,
+                           A, B, C, nullptr, nullptr, nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_muon_upper(int n, int batch, const MspSystem *systems,
+                                  const int32_t *sys_id, const double *vec, int pauli,
+                                  const double *U, double *O, hipStream_t s)
+{
+    ProfScope ps_("k_gemm_cn", s);
+    const unsigned t = (unsigned)((n + 63) / 64);
+    hipLaunchKernelGGL((k_gemm_cn_mfma<true, true>), dim3(t, t, batch), dim3(256), 0, s, n, U, U,
+                       O, systems, sys_id, vec, pauli);
     return hipGetLastError();
 }

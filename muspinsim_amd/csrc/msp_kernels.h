@@ -114,3 +114,6 @@ hipError_t launch_gemm_cn_upper(int n, int batch, const double *A, const double 
 hipError_t launch_integral_obs(int n, int batch, const double *E, const double *O, double alpha,
                                double beta, double tau, double out_scale, const double *w,
                                double *out, hipStream_t s);
+hipError_t launch_gemm_muon_upper(int n, int batch, const MspSystem *systems,
+                                  const int32_t *sys_id, const double *vec, int pauli,
+                                  const double *U, double *O, hipStream_t s);

@@ -325,6 +325,8 @@ class SpinEngine:
                 if route == "fast":
                     s0 = systems[int(batch.sys_id[idx[0]])]
                     g.d_other = n // s0.dims[s0.muon_index]
+                g.mu_half = all(systems[int(k)].dims[systems[int(k)].muon_index] == 2
+                                for k in np.unique(batch.sys_id[idx]))
                 # systems shared by many instances get their field-independent
                 # H_sys built once per execute; H = H_sys + Zeeman per instance
                 uniq, local = np.unique(batch.sys_id[idx], return_inverse=True)
@@ -429,20 +431,28 @@ class SpinEngine:
                 plan.checks.append(g.info[:nb].amax())
                 Y = g.H   # destroyed by the eigensolver; reused as scratch
                 pauli = 1 if g.route == "fast" else 0
-                N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
-                                            g.pr.data_ptr(), pauli, g.U.data_ptr(), Y.data_ptr(), st))
                 out = torch.empty((nb, plan.width), dtype=torch.float64, device=self.device)
                 if batch.y == "integral" and g.route in ("fast", "iso"):
                     # O is Hermitian and the integral weights |O_ij|^2 by an
                     # even kernel: the upper block triangle of O suffices
-                    N.check(L.msp_gemm_cn_upper(n, nb, g.U.data_ptr(), Y.data_ptr(),
-                                                g.O.data_ptr(), st))
+                    if g.mu_half:
+                        N.check(L.msp_gemm_muon_upper(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                                      g.pr.data_ptr(), pauli, g.U.data_ptr(),
+                                                      g.O.data_ptr(), st))
+                    else:
+                        N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                                    g.pr.data_ptr(), pauli, g.U.data_ptr(),
+                                                    Y.data_ptr(), st))
+                        N.check(L.msp_gemm_cn_upper(n, nb, g.U.data_ptr(), Y.data_ptr(),
+                                                    g.O.data_ptr(), st))
                     alpha, beta = ((1.0 / (4.0 * g.d_other), 0.0) if g.route == "fast"
                                    else (2.0 / n, 1.0 / n))
                     N.check(L.msp_integral_obs(n, nb, g.E.data_ptr(), g.O.data_ptr(), alpha, beta,
                                                MU_TAU, 1.0 / MU_TAU, p(g.w), out.data_ptr(), st))
                     plan.rows.index_copy_(0, g.d_idx[sl], out)
                     continue
+                N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                            g.pr.data_ptr(), pauli, g.U.data_ptr(), Y.data_ptr(), st))
                 N.check(L.msp_gemm_cn(n, nb, g.U.data_ptr(), Y.data_ptr(), g.O.data_ptr(), st))
                 if g.route == "fast":
                     N.check(L.msp_form_c(n, nb, None, g.O.data_ptr(), 1, 1.0 / (4.0 * g.d_other),

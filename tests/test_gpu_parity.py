@@ -199,6 +199,53 @@ def test_upper_block_gemm_and_observable_integral(eng, n):
             assert abs(got[b] - npref) <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("dims,mu", [((2, 3, 2), 0), ((3, 2, 2, 4), 1), ((4, 2, 2, 6), 2),
+                                     ((2, 96), 0), ((12, 2, 8), 1)])
+def test_muon_mixing_upper_block_gemm(eng, dims, mu):
+    """msp_gemm_muon_upper equals msp_apply_muon_op + msp_gemm_cn_upper on the
+    upper blocks, and U^H (f v.S_mu (x) 1) U built with numpy, for a spin-1/2
+    muon at any position and both the Pauli (f = 2) and spin (f = 1) forms."""
+    from muspinsim_amd import _native as N
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n + mu)
+    nb = 4
+    _, U = np.linalg.eigh(np.stack([_herm(rng, n) for _ in range(nb)]))
+    vec = rng.normal(size=(nb, 3))
+    sysarr = np.zeros(2, dtype=N.SYSTEM_DTYPE)
+    for k in range(2):
+        sysarr[k]["n_spins"] = len(dims)
+        sysarr[k]["n"] = n
+        sysarr[k]["muon_index"] = mu
+        sysarr[k]["dims"][: len(dims)] = dims
+    dsys = eng._bytes_view(sysarr)
+    sid = _dev(np.array([1, 0, 1, 0], dtype=np.int32), torch.int32)
+    dU, dv = _dev(U), _dev(vec, torch.float64)
+    st = eng.stream()
+    sx = np.array([[0, 0.5], [0.5, 0]]); sy = np.array([[0, -0.5j], [0.5j, 0]])
+    sz = np.diag([0.5, -0.5])
+    blk = np.arange(n) // 64
+    mask = blk[:, None] <= blk[None, :]
+    for pauli in (1, 0):
+        Y = torch.empty_like(dU)
+        ref = torch.empty_like(dU)
+        got = torch.full_like(dU, 7.0)
+        N.check(eng.lib.msp_apply_muon_op(n, nb, dsys.data_ptr(), sid.data_ptr(), dv.data_ptr(),
+                                          pauli, dU.data_ptr(), Y.data_ptr(), st))
+        N.check(eng.lib.msp_gemm_cn_upper(n, nb, dU.data_ptr(), Y.data_ptr(), ref.data_ptr(), st))
+        N.check(eng.lib.msp_gemm_muon_upper(n, nb, dsys.data_ptr(), sid.data_ptr(), dv.data_ptr(),
+                                            pauli, dU.data_ptr(), got.data_ptr(), st))
+        r, g = ref.cpu().numpy(), got.cpu().numpy()
+        assert np.all(g[:, ~mask] == 7.0)
+        f = 2.0 if pauli else 1.0
+        for b in range(nb):
+            m = f * (vec[b, 0] * sx + vec[b, 1] * sy + vec[b, 2] * sz)
+            op = np.kron(np.kron(np.eye(int(np.prod(dims[:mu]))), m),
+                         np.eye(int(np.prod(dims[mu + 1:]))))
+            npref = U[b].conj().T @ op @ U[b]
+            assert np.abs(g[b][mask] - r[b][mask]).max() <= 1e-13 * f * np.abs(vec[b]).sum()
+            assert np.abs(g[b][mask] - npref[mask]).max() <= 1e-12 * f * np.abs(vec[b]).sum()
+
+
 def test_timesum_and_integral(eng):
     rng = np.random.default_rng(3)
     for n, T in ((4, 7), (16, 130), (70, 65)):
