@@ -26,7 +26,7 @@ constexpr int kWaves = 4;
 #define MSP_TD_ROWS16 1 // n <= 192: A22 v with one row per 16-lane group
 #endif
 #ifndef MSP_TD_RP
-#define MSP_TD_RP 1     // rows per 16-lane group in one pass of that product
+#define MSP_TD_RP 2     // rows per 16-lane group in one pass of that product
 #endif
 
 // reductions across a wave run on DPP lane moves (msp_team.h)
