@@ -10,7 +10,8 @@
 //     own columns; a row's left part is reduced across the wave (row sum)
 //     while conj(a_ic) v_i accumulates per lane (column sum), so every
 //     element is read once per step;
-//   - the end-of-panel update A -= Vp Wp^H + Wp Vp^H touches c <= i only.
+//   - the end-of-panel update A -= Vp Wp^H + Wp Vp^H touches c <= i only,
+//     with each lane's column of Vp and Wp held in registers.
 #include <hip/hip_runtime.h>
 
 #include "msp_kernels.h"
@@ -19,6 +20,7 @@
 namespace {
 
 constexpr int kWaves = 4;
+constexpr int kPanelRows = 8;   // rows per batched pass of the end-of-panel update
 #ifndef MSP_TD_RB
 #define MSP_TD_RB 4     // rows per batched pass of the A22 v product (64-lane rows)
 #endif
@@ -126,66 +128,52 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
     }
 }
 
-// One pass of the end-of-panel rank-2k update over rows i0 and i0 + 4 of
-// this wave, columns t0 + lane + 64 q for q < QA (compile-time, so the loads
-// of the pass are all issued before the first use):
-// A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl) for c <= i.
-template <int NB, int QMAX, int QA>
-__device__ inline void panel_pass(int n, int t0, int i0, int lane, cdbl *A, const cdbl *Vp,
-                                  const cdbl *Wp, cdbl *col)
+// End-of-panel rank-2k update with lanes owning columns: for each 64-column
+// block the lane's column of Vp and Wp stays in registers while the wave walks
+// the rows at or below the block (rows wave, wave + 4, ..., RB per pass, all
+// loads of a pass issued first); a row's Vp and Wp entries are wave-uniform
+// LDS reads.  A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl), c <= i.
+// (always_inline: as a call, the ABI's saved registers spill to scratch)
+template <int NB, int RB>
+__device__ __attribute__((always_inline)) inline void panel_update_cols(
+    int n, int t0, int wave, int lane, cdbl *A, const cdbl *Vp, const cdbl *Wp, cdbl *col)
 {
     constexpr int LDP = NB + 1;
-    cdbl av[2][QA];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int i = i0 + u * kWaves;
-        const int ic = (i < n) ? i : n - 1;
-#pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            const int c = t0 + lane + 64 * q;
-            av[u][q] = A[(size_t)ic * n + ((c <= ic) ? c : ic)];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int i = i0 + u * kWaves;
-        if (i >= n) continue;
-        cdbl vr[NB], wr[NB];
+    for (int cb = t0; cb < n; cb += 64) {
+        const int c = cb + lane;
+        const bool cok = c < n;
+        const int cc = cok ? c : n - 1;
+        cdbl vc[NB], wc[NB];
 #pragma unroll
         for (int l = 0; l < NB; ++l) {
-            vr[l] = Vp[i * LDP + l];
-            wr[l] = Wp[i * LDP + l];
+            vc[l] = cconj(Vp[cc * LDP + l]);
+            wc[l] = cconj(Wp[cc * LDP + l]);
         }
+        for (int i0 = cb + wave; i0 < n; i0 += RB * kWaves) {
+            cdbl av[RB];
 #pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            const int c = t0 + lane + 64 * q;
-            if (c <= i) {
-                cdbl a = av[u][q];
+            for (int u = 0; u < RB; ++u) {
+                const int i = i0 + u * kWaves;
+                const int ic = (i < n) ? i : n - 1;
+                av[u] = A[(size_t)ic * n + ((cc <= ic) ? cc : ic)];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int i = i0 + u * kWaves;
+                if (i >= n) break;                       // wave-uniform
+                cdbl a = av[u];
 #pragma unroll
                 for (int l = 0; l < NB; ++l) {
-                    const cdbl vc = Vp[c * LDP + l], wc = Wp[c * LDP + l];
-                    a = csub(a, cmul(vr[l], cconj(wc)));
-                    a = csub(a, cmul(wr[l], cconj(vc)));
+                    a = csub(a, cmul(Vp[i * LDP + l], wc[l]));
+                    a = csub(a, cmul(Wp[i * LDP + l], vc[l]));
                 }
-                A[(size_t)i * n + c] = a;
-                if (c == t0) col[i] = a;                  // next step's column
+                if (cok && c <= i) {
+                    A[(size_t)i * n + c] = a;
+                    if (c == t0) col[i] = a;             // next step's column
+                }
             }
         }
     }
-}
-
-// Calls panel_pass with QA = qact (1 <= qact <= QA).
-template <int NB, int QMAX, int QA>
-__device__ inline void panel_dispatch(int qact, int n, int t0, int i0, int lane, cdbl *A,
-                                      const cdbl *Vp, const cdbl *Wp, cdbl *col)
-{
-    if constexpr (QA > 1) {
-        if (qact < QA) {
-            panel_dispatch<NB, QMAX, QA - 1>(qact, n, t0, i0, lane, A, Vp, Wp, col);
-            return;
-        }
-    }
-    panel_pass<NB, QMAX, QA>(n, t0, i0, lane, A, Vp, Wp, col);
 }
 
 template <int NB, int QMAX, int RB, int Q16, bool GP>
@@ -398,12 +386,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
             const int i = idx / nbk, l = idx % nbk;
             if (i >= k0 + l + 2) A[(size_t)i * n + k0 + l] = Vp[i * LDP + l];
         }
-        const int t0 = k0 + nbk;
-        for (int i0 = t0 + wave; i0 < n; i0 += 2 * kWaves) {
-            const int ilast = (i0 + kWaves < n) ? i0 + kWaves : n - 1;
-            const int qact = ((ilast - t0) >> 6) + 1;           // wave-uniform, <= QMAX
-            panel_dispatch<NB, QMAX, QMAX>(qact, n, t0, i0, lane, A, Vp, Wp, col);
-        }
+        panel_update_cols<NB, kPanelRows>(n, k0 + nbk, wave, lane, A, Vp, Wp, col);
         __syncthreads();
     }
     if (tid == 0) d[n - 1] = col[n - 1].re;
