@@ -214,7 +214,9 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
         for (int j = 0; j < nbk; ++j) {
             const int k = k0 + j;
             const int m = n - k - 1;
-            // (a) column k, rows k..n-1, brought up to date within the panel
+            // (a) column k, rows k..n-1, brought up to date within the panel,
+            // with the squared norm of rows k+2.. summed on the way
+            double part = 0.0;
             if (j > 0) {
                 for (int i = k + tid; i < n; i += 256) {
                     cdbl a = col[i];
@@ -223,12 +225,12 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                         a = csub(a, cmul(Wp[i * LDP + l], cconj(Vp[k * LDP + l])));
                     }
                     col[i] = a;
+                    if (i >= k + 2) part += cabs2(a);
                 }
-                __syncthreads();
+            } else {
+                for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(col[i]);
             }
             // (b) reflector of A[k+1.., k]
-            double part = 0.0;
-            for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(col[i]);
             part = wave_sum(part);
             if (lane == 0) red[wave] = part;
             __syncthreads();
@@ -352,7 +354,8 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 }
                 pre = wave_sum(pre);
                 pim = wave_sum(pim);
-                __syncthreads();   // red reuse
+                // (red was last read before the barrier that follows the
+                // v write, so it can be overwritten here)
                 if (lane == 0) {
                     red[wave] = pre;
                     red[4 + wave] = pim;
