@@ -297,8 +297,10 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 // Divide and conquer on one tridiagonal per workgroup.  The n-sized arrays of
 // the merges live in LDS when `small_in_lds` (dynamic LDS holds them), the
 // n x n matrices (Q0, Q1, S, QK) in the global workspace.
-__global__ void __launch_bounds__(256) k_stedc(int n, double *E, double *work, int32_t *info,
-                                               int small_in_lds)
+// (four waves per SIMD: the compiler fits the kernel in 128 registers
+// without spilling, so four instances share a CU instead of three)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_stedc(int n, double *E, double *work, int32_t *info, int small_in_lds)
 {
     __shared__ double red[256];
     extern __shared__ __align__(16) unsigned char smem[];
