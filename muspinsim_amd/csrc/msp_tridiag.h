@@ -490,21 +490,77 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
             }
         }
         MSP_SYNC();
-        // --- deflation scan, one thread per block ---
+        // --- deflation tolerance, one thread per block (kept in item_v[lo],
+        // which is not read before the merge items below) ---
+        MSP_PFOR(lo0, n)
+        {
+            const int lo = W.blo[lo0], hi = W.bhi[lo0];
+            if (lo == lo0 && hi - lo >= 2) {
+                double dmax = 0.0, zmax = 0.0;
+                for (int s = lo; s < hi; ++s) {
+                    dmax = fmax(dmax, fabs(W.ds[s]));
+                    zmax = fmax(zmax, fabs(W.zs[s]));
+                }
+                W.item_v[lo] = 8.0 * eps * fmax(dmax, zmax);
+            }
+        }
+        MSP_SYNC();
+        // --- per slot: z deflation (defl) and whether the slot would rotate
+        // with the previous undeflated slot (korg, rewritten by the secular
+        // step), both as the scan below tests them while no rotation has
+        // happened yet in the block ---
+        MSP_PFOR(s, n)
+        {
+            const int lo = W.blo[s], hi = W.bhi[s];
+            if (hi - lo >= 2) {
+                const int mid = lo + (hi - lo) / 2;
+                const double rho = 2.0 * fabs(e[mid - 1] * inv);
+                const double tol = W.item_v[lo];
+                const int zd = (rho * fabs(W.zs[s]) <= tol) ? 1 : 0;
+                int rot = 0;
+                if (!zd) {
+                    int pj = s - 1;
+                    while (pj >= lo && rho * fabs(W.zs[pj]) <= tol) --pj;
+                    if (pj >= lo) {
+                        double zs_ = W.zs[pj], zc = W.zs[s];
+                        double tu = sqrt(zs_ * zs_ + zc * zc);
+                        double c = zc / tu, sn = -zs_ / tu;
+                        double t = W.ds[s] - W.ds[pj];
+                        rot = (fabs(t * c * sn) <= tol) ? 1 : 0;
+                    }
+                }
+                W.defl[s] = zd;
+                W.korg[s] = rot;
+            }
+        }
+        MSP_SYNC();
+        // --- deflation scan, one thread per block: the slots before the
+        // first rotation take their precomputed flags; the scan tests
+        // (and rotates) from there on ---
         MSP_PFOR(lo0, n)
         {
             const int lo = W.blo[lo0], hi = W.bhi[lo0];
             if (lo == lo0 && hi - lo >= 2) {
                 int mid = lo + (hi - lo) / 2;
                 double rho = 2.0 * fabs(e[mid - 1] * inv);
-                double dmax = 0.0, zmax = 0.0;
-                for (int s = lo; s < hi; ++s) {
-                    dmax = fmax(dmax, fabs(W.ds[s]));
-                    zmax = fmax(zmax, fabs(W.zs[s]));
-                }
-                double tol = 8.0 * eps * fmax(dmax, zmax);
+                const double tol = W.item_v[lo];
                 int K = 0, nr = 0, nd = 0, pj = -1;
-                for (int s = lo; s < hi; ++s) {
+                int s0 = lo;
+                for (; s0 < hi; ++s0) {
+                    if (W.defl[s0]) {
+                        W.dfl[lo + nd++] = s0;
+                        continue;
+                    }
+                    if (W.korg[s0]) break;
+                    if (pj >= 0) {
+                        W.dl[lo + K] = W.ds[pj];
+                        W.zl[lo + K] = W.zs[pj];
+                        W.kcol[lo + K] = W.perm[pj];
+                        ++K;
+                    }
+                    pj = s0;
+                }
+                for (int s = s0; s < hi; ++s) {
                     W.defl[s] = 0;
                     if (rho * fabs(W.zs[s]) <= tol) {
                         W.defl[s] = 1;
