@@ -108,6 +108,15 @@ int msp_build_h(int n, int batch, const MspSystem *d_systems, const MspTerm *d_t
                 const int32_t *d_sys_id, const double *d_Br, const double *d_base,
                 int zeeman, double *d_H, void *stream);
 
+/* msp_build_h writing only the lower triangle (column <= row) of each H[b];
+ * the strict upper triangle is left as it was.  msp_eigh with MSP_EIGH_DC
+ * (or AUTO resolving to it) and n <= 1024 reads only that triangle, so this
+ * halves the matrix writes in front of it (same reference interfaces as
+ * msp_build_h). */
+int msp_build_h_lower(int n, int batch, const MspSystem *d_systems, const MspTerm *d_terms,
+                      const int32_t *d_sys_id, const double *d_Br, const double *d_base,
+                      int zeeman, double *d_H, void *stream);
+
 /* Batched Hermitian eigendecomposition.  d_H is destroyed.  d_E: batch x n
  * ascending eigenvalues; d_U: batch x n x n eigenvectors (columns).
  * d_info: batch int32, 0 = converged. */

@@ -51,6 +51,7 @@ SIGNATURES = {
     "msp_last_error": (_I, [ctypes.c_char_p, _SZ]),
     "msp_rotate_fields": (_I, [_I, _P, _P, _P, _P, _P, _P, _P]),
     "msp_build_h": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "msp_build_h_lower": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _P, _P]),
     "msp_eigh_workspace": (_SZ, [_I, _I, _I]),
     "msp_eigh": (_I, [_I, _I, _P, _P, _P, _P, _I, _P, _SZ, _P]),
     "msp_apply_muon_op": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),

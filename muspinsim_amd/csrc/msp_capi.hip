@@ -116,8 +116,23 @@ int msp_build_h(int n, int batch, const MspSystem *d_systems, const MspTerm *d_t
     if (!d_systems || !d_H || (zeeman && !d_Br) || (!d_base && !d_terms))
         return fail(MSP_ERR_VALUE, "msp_build_h: null pointer");
     return hip_status(launch_build_h(n, batch, d_systems, d_terms, d_sys_id, d_Br, d_base, zeeman,
-                                     d_H, S(stream)),
+                                     0, d_H, S(stream)),
                       "msp_build_h");
+}
+
+int msp_build_h_lower(int n, int batch, const MspSystem *d_systems, const MspTerm *d_terms,
+                      const int32_t *d_sys_id, const double *d_Br, const double *d_base,
+                      int zeeman, double *d_H, void *stream)
+{
+    if (n < 1 || n > kMaxDim)
+        return fail(MSP_ERR_VALUE, "msp_build_h_lower: n = %d out of range", n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_build_h_lower: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_systems || !d_H || (zeeman && !d_Br) || (!d_base && !d_terms))
+        return fail(MSP_ERR_VALUE, "msp_build_h_lower: null pointer");
+    return hip_status(launch_build_h(n, batch, d_systems, d_terms, d_sys_id, d_Br, d_base, zeeman,
+                                     1, d_H, S(stream)),
+                      "msp_build_h_lower");
 }
 
 size_t msp_eigh_workspace(int n, int batch, int method)

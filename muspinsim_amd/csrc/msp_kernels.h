@@ -33,7 +33,7 @@ hipError_t launch_rotate(int batch, const double *q, const double *B, const doub
                          const double *p, double *Br, double *pr, hipStream_t s);
 hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspTerm *terms,
                           const int32_t *sys_id, const double *Br, const double *base, int zeeman,
-                          double *H, hipStream_t s);
+                          int lower, double *H, hipStream_t s);
 hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, double *U,
                               int32_t *info, int max_sweeps, hipStream_t s);
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,

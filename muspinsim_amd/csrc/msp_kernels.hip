@@ -142,11 +142,12 @@ __device__ cdbl h_element(int n, const MspSystem &S, const MspTerm *terms, const
 // per element, index digits by division (any multiplicities).
 __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
                           const int32_t *sys_id, const double *Br, const double *base,
-                          int zeeman, double *H)
+                          int zeeman, int lower, double *H)
 {
     const int b = blockIdx.y;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * n) return;
+    if (lower && idx % n > idx / n) return;
     const int s = sys_id ? sys_id[b] : 0;
     const cdbl *bm = base ? (const cdbl *)base + (size_t)s * n * n : nullptr;
     ((cdbl *)H)[(size_t)b * n * n + idx] = h_element(n, systems[s], terms, bm, Br + 3 * b, zeeman, idx);
@@ -161,7 +162,8 @@ __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
 constexpr int kBuildEPT = 4;
 __global__ void __launch_bounds__(256) k_build_h_base(int n, const MspSystem *systems,
                                                       const MspTerm *terms, const int32_t *sys_id,
-                                                      const double *Br, const double *base, double *H)
+                                                      const double *Br, const double *base,
+                                                      int lower, double *H)
 {
     extern __shared__ unsigned long long digs[];
     const int b = blockIdx.y;
@@ -173,12 +175,24 @@ __global__ void __launch_bounds__(256) k_build_h_base(int n, const MspSystem *sy
     const cdbl *bm = (const cdbl *)base + (size_t)s * n * n;
     cdbl *Hb = (cdbl *)H + (size_t)b * n * n;
     const double *B = Br + 3 * b;
-    const int nn = n * n;
-    const int idx0 = blockIdx.x * 256 * kBuildEPT + threadIdx.x;
+    // element t of the launch: row-major over the matrix, or (lower) over
+    // its packed lower triangle, row r holding t = r (r + 1) / 2 + c
+    const int cnt = lower ? n * (n + 1) / 2 : n * n;
+    const int t0 = blockIdx.x * 256 * kBuildEPT + threadIdx.x;
+    auto element = [&](int t) {
+        if (!lower) return t;
+        int r = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((r + 1) * (r + 2) / 2 <= t) ++r;
+        while (r * (r + 1) / 2 > t) --r;
+        return r * n + (t - r * (r + 1) / 2);
+    };
     if (!packable) {
         for (int e = 0; e < kBuildEPT; ++e) {
-            const int idx = idx0 + e * 256;
-            if (idx < nn) Hb[idx] = h_element(n, S, terms, bm, B, 1, idx);
+            const int t = t0 + e * 256;
+            if (t < cnt) {
+                const int idx = element(t);
+                Hb[idx] = h_element(n, S, terms, bm, B, 1, idx);
+            }
         }
         return;
     }
@@ -194,8 +208,9 @@ __global__ void __launch_bounds__(256) k_build_h_base(int n, const MspSystem *sy
     }
     __syncthreads();
     for (int e = 0; e < kBuildEPT; ++e) {
-        const int idx = idx0 + e * 256;
-        if (idx >= nn) break;
+        const int t = t0 + e * 256;
+        if (t >= cnt) break;
+        const int idx = element(t);
         const int r = idx / n, c = idx - r * n;
         cdbl acc = bm[idx];
         const unsigned long long dr = digs[r], x = dr ^ digs[c];
@@ -918,16 +933,17 @@ hipError_t launch_rotate(int batch, const double *q, const double *B, const doub
 
 hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspTerm *terms,
                           const int32_t *sys_id, const double *Br, const double *base, int zeeman,
-                          double *H, hipStream_t s)
+                          int lower, double *H, hipStream_t s)
 {
     ProfScope ps_("k_build_h", s);
     if (base && zeeman) {
-        hipLaunchKernelGGL(k_build_h_base, dim3(cdiv((size_t)n * n, 256 * kBuildEPT), batch),
+        const size_t cnt = lower ? (size_t)n * (n + 1) / 2 : (size_t)n * n;
+        hipLaunchKernelGGL(k_build_h_base, dim3(cdiv(cnt, 256 * kBuildEPT), batch),
                            dim3(256), sizeof(unsigned long long) * n, s, n, systems, terms, sys_id,
-                           Br, base, H);
+                           Br, base, lower, H);
     } else {
         hipLaunchKernelGGL(k_build_h, dim3(cdiv((size_t)n * n, 256), batch), dim3(256), 0, s, n,
-                           systems, terms, sys_id, Br, base, zeeman, H);
+                           systems, terms, sys_id, Br, base, zeeman, lower, H);
     }
     return hipGetLastError();
 }

@@ -151,6 +151,14 @@ class SpinEngine:
         return int(max(1, min(self.max_chunk, self.chunk_bytes // per)))
 
     # ---------------------------------------------------------------- stages
+    def _dc_lower(self, n):
+        """Whether msp_eigh with this engine's method reads only the lower
+        triangle of an n x n input (the D&C path up to n = 1024)."""
+        m = self.method
+        if m == N.MSP_EIGH_AUTO:
+            m = N.MSP_EIGH_JACOBI if n <= 64 else N.MSP_EIGH_DC
+        return m == N.MSP_EIGH_DC and n <= 1024
+
     def eigh(self, H, method=None):
         """E (B, n) ascending and U (B, n, n) of a batch of Hermitian matrices
         (complex128, destroyed)."""
@@ -416,13 +424,16 @@ class SpinEngine:
 
                 N.check(L.msp_rotate_fields(nb, p(g.q), p(g.B), p(g.Bint), p(g.p),
                                             g.Br.data_ptr(), g.pr.data_ptr(), st))
+                # the D&C eigensolver (n <= 1024) reads the lower triangle only
+                build = (L.msp_build_h_lower if g.route != "lindblad" and self._dc_lower(n)
+                         else L.msp_build_h)
                 if g.base is not None:
-                    N.check(L.msp_build_h(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
-                                          p(g.sid_local), g.Br.data_ptr(), g.base.data_ptr(), 1,
-                                          g.H.data_ptr(), st))
+                    N.check(build(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
+                                  p(g.sid_local), g.Br.data_ptr(), g.base.data_ptr(), 1,
+                                  g.H.data_ptr(), st))
                 else:
-                    N.check(L.msp_build_h(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
-                                          p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
+                    N.check(build(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
+                                  p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
                 if g.route == "lindblad":
                     self._execute_lindblad(plan, g, sl, nb, st)
                     continue

@@ -146,6 +146,17 @@ def test_build_h_from_base(eng, Is):
         ref = physics.hamiltonian(S.Is, S.terms) + physics.zeeman(S.Is, S.gammas, Brh[k])
         assert np.abs(H1[k] - ref).max() <= 1e-12 * np.abs(ref).max()
         assert np.abs(H2[k] - ref).max() <= 1e-12 * np.abs(ref).max()
+    # msp_build_h_lower: the same lower triangles (both kernels), the strict
+    # upper triangle untouched
+    low = np.tril(np.ones((n, n), bool))
+    for base_ptr, terms_ptr, full in ((None, dterms.data_ptr(), H1),
+                                      (base.data_ptr(), None, H2)):
+        H3 = torch.full((nb, n, n), 7.0 + 7.0j, dtype=torch.complex128, device="cuda")
+        N.check(L.msp_build_h_lower(n, nb, dsys.data_ptr(), terms_ptr, sid.data_ptr(),
+                                    Br.data_ptr(), base_ptr, 1, H3.data_ptr(), eng.stream()))
+        H3 = H3.cpu().numpy()
+        assert np.array_equal(H3[:, low], full[:, low])
+        assert np.all(H3[:, ~low] == 7.0 + 7.0j)
 
 
 def test_gemm_cn(eng):
