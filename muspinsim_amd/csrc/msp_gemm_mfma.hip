@@ -20,11 +20,12 @@ namespace {
 // the stride its system gives: B[k] = M[a][a] A[k] + M[a][1-a] A[k'] with a
 // the muon digit of row k and k' the row with the other digit
 // (the product k_apply_muon writes, f = 2 for Pauli matrices).
+// (three waves per SIMD: the accumulators then stay in VGPRs, 149 registers
+// with MIX, no spills)
 template <bool UPPER, bool MIX>
-__global__ void __launch_bounds__(256) k_gemm_cn_mfma(int n, const double *A, const double *B,
-                                                      double *C, const MspSystem *systems,
-                                                      const int32_t *sys_id, const double *vec,
-                                                      int pauli)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSystem *systems,
+               const int32_t *sys_id, const double *vec, int pauli)
 {
     if (UPPER && blockIdx.y > blockIdx.x) return;
     const int b = blockIdx.z;
