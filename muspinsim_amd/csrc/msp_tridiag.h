@@ -206,10 +206,11 @@ MSP_HD double msp_recip(double x)
 #endif
 }
 
-MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho, int j,
-                           int &org_out)
+MSP_HD double secular_root(int K, const double *dl, const double *zl, const double *zz,
+                           double rho, int j, int &org_out)
 {
-    // Root j of 1 + rho sum_i zl_i^2/(dl_i - x) = 0, dl ascending, rho > 0.
+    // Root j of 1 + rho sum_i zl_i^2/(dl_i - x) = 0, dl ascending, rho > 0;
+    // zz_i = rho zl_i zl_i (same rounding as the product written out).
     const double eps = 2.220446049250313e-16;
     int org;
     double lo, hi;
@@ -223,7 +224,7 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
         double mid = 0.5 * gap;
         for (int i = 0; i < K; ++i) {
             const double r = msp_recip((dl[i] - dl[j]) - mid);
-            const double t = rho * zl[i] * zl[i] * r;
+            const double t = zz[i] * r;
             if (i <= j) { m_psi += t; m_dpsi += t * r; }
             else { m_phi += t; m_dphi += t * r; }
         }
@@ -238,7 +239,7 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
         hi = rho * z2;
     }
     org_out = org;
-    if (K == 1) return rho * zl[0] * zl[0];
+    if (K == 1) return zz[0];
     const double dorg = dl[org];
     // interior model uses the two poles j and j+1 around the root
     const int pl = (j < K - 1) ? j : K - 1;
@@ -257,7 +258,7 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, double rho
         } else {
             for (int i = 0; i < K; ++i) {
                 const double r = msp_recip((dl[i] - dorg) - tau);
-                const double t = rho * zl[i] * zl[i] * r;
+                const double t = zz[i] * r;
                 if (i <= pl) { psi += t; dpsi += t * r; }
                 else { phi += t; dphi += t * r; }
             }
@@ -634,8 +635,15 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     Q[(size_t)a * n + row] = c * qa + s * qb;
                     Q[(size_t)b * n + row] = c * qb - s * qa;
                 }
+                // rho z^2 of the kept poles for the secular solves (zh is
+                // written with the Loewner z only after them)
+                if (row - lo < W.nkept[lo]) {
+                    const double rho = 2.0 * fabs(e[mid - 1] * inv);
+                    W.zh[row] = rho * W.zl[row] * W.zl[row];
+                }
             }
         }
+        MSP_SYNC();
         // --- secular roots ---
         MSP_PFOR(s, n)
         {
@@ -647,7 +655,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 int j = s - lo;
                 if (j < K) {
                     int org;
-                    double t = secular_root(K, W.dl + lo, W.zl + lo, rho, j, org);
+                    double t = secular_root(K, W.dl + lo, W.zl + lo, W.zh + lo, rho, j, org);
                     W.korg[s] = org;
                     W.ktau[s] = t;
                     // product order of the kept columns: first child only,
