@@ -20,12 +20,15 @@
  * Reference interfaces replaced (paths in the muspinsim tree, v2.3.1):
  *   msp_rotate_fields   ExperimentRunner.load_config rotations, experiment.py:384-411
  *   msp_build_h         SpinSystem.hamiltonian + ExperimentRunner.Hz/Htot,
- *                       spinsys.py:34-69,613-626; experiment.py:238-269,331-341
+ *   msp_build_h_lower   spinsys.py:34-69,613-626; experiment.py:238-269,331-341
  *   msp_eigh            Hermitian.diag -> numpy.linalg.eigh, spinop.py:51-82
  *   msp_apply_muon_op   MuonSpinSystem.muon_operator / sigma_mu (x) 1,
  *                       spinsys.py:707-760; hamiltonian.py:204
  *   msp_apply_kron      DensityOperator kron of per-spin states, experiment.py:169-236
  *   msp_gemm_cn         Operator.basis_change (U^H M U), spinop.py:330-355
+ *   msp_gemm_cn_upper,  the same basis change of sigma_mu on the integral path, blocks
+ *   msp_gemm_muon_upper on or above the diagonal only, experiment.py:486-496
+ *   msp_integral_obs    integrate_decaying of |O_ij|^2 (fast_evolve's C), hamiltonian.py:150-162
  *   msp_form_c          rho' * O'^T of Hamiltonian.evolve, hamiltonian.py:77-103;
  *                       |U^H sigma U|^2 of fast_evolve, hamiltonian.py:207-210
  *   msp_timesum         evolve time loop, hamiltonian.py:87-107, and the
@@ -45,6 +48,9 @@
  *                       fitting.py:99-105)
  *   msp_accumulate      Re(data)*w and store_time_slice powder accumulation,
  *                       experiment.py:498; simconfig.py:347-368
+ *   msp_celio_evolve    CelioHamiltonian.evolve / fast_evolve Trotter loop,
+ *                       celio.py:210-293,318-476
+ *   msp_graph_*, msp_profile_*  no counterpart: HIP graph capture and kernel timing
  */
 #ifndef MUSPIN_HIP_H
 #define MUSPIN_HIP_H
