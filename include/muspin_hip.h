@@ -157,7 +157,9 @@ int msp_gemm_cn_upper(int n, int batch, const double *d_A, const double *d_B, do
  * diagonal (as msp_gemm_cn_upper), with the muon operator applied while the
  * second operand is loaded: the product msp_apply_muon_op then msp_gemm_cn
  * form (experiment.py:486-496 via Operator.basis_change, spinop.py:330-355)
- * in one pass for a spin-1/2 muon; f = 2 (Pauli) when pauli != 0. */
+ * in one pass for a spin-1/2 muon; f = 2 (Pauli) when pauli != 0.  An
+ * instance whose system's muon has another multiplicity gets NaN blocks
+ * (use msp_apply_muon_op + msp_gemm_cn_upper for it). */
 int msp_gemm_muon_upper(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
                         const double *d_vec, int pauli, const double *d_U, double *d_O,
                         void *stream);

@@ -30,9 +30,12 @@ k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSyst
     if (UPPER && blockIdx.y > blockIdx.x) return;
     const int b = blockIdx.z;
     int mstride = 1;
+    bool mix_ok = true;
     cdbl m_same[2], m_other[2];
     if constexpr (MIX) {
         const MspSystem &S = systems[sys_id ? sys_id[b] : 0];
+        mix_ok = S.dims[S.muon_index] == 2;   // otherwise the block writes NaN
+        if (!mix_ok) mstride = 0x40000000;    // (keeps every partner row in range)
         for (int k = S.n_spins - 1; k > S.muon_index; --k) mstride *= S.dims[k];
         const double f = pauli ? 1.0 : 0.5;        // v.sigma or v.S
         const double vx = vec[3 * b], vy = vec[3 * b + 1], vz = vec[3 * b + 2];
@@ -68,7 +71,7 @@ k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSyst
             av[h][1] = Ab[(size_t)kc * n + ibc];
             if constexpr (MIX) {
                 const int a = (kc / mstride) & 1;
-                const int k2 = kc + (a ? -mstride : mstride);
+                const int k2 = mix_ok ? kc + (a ? -mstride : mstride) : kc;
                 const cdbl x0 = Bb[(size_t)kc * n + jac], x1 = Bb[(size_t)kc * n + jbc];
                 const cdbl y0 = Bb[(size_t)k2 * n + jac], y1 = Bb[(size_t)k2 * n + jbc];
                 bv[h][0] = cadd(cmul(m_same[a], x0), cmul(m_other[a], y0));
@@ -104,7 +107,9 @@ k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSyst
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = i0 + 16 * x + 4 * q + lq, j = j0 + 16 * y + li;
-                if (i < n && j < n) Cb[(size_t)i * n + j] = cmk(cr[x][y][q], ci[x][y][q]);
+                if (i < n && j < n)
+                    Cb[(size_t)i * n + j] = mix_ok ? cmk(cr[x][y][q], ci[x][y][q])
+                                                   : cmk(__builtin_nan(""), __builtin_nan(""));
             }
 }
 

@@ -770,26 +770,33 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
         // the products Q[:, kcol] S below.  Rows of the parent block outside
         // the column's block keep stale values, which the next level never
         // reads (its supports).
+        // (items are (column, row lane) pairs with at most 64 lanes per
+        // column, each lane striding over its block's rows, so a kept
+        // column costs one item per lane at every level)
         {
             const int smax = dc_maxblock(n, L);
             int sh = 0;
             while ((1 << sh) < smax) ++sh;
-            MSP_PFOR(idx, n << sh)
+            const int cs = (sh > 6) ? 6 : sh;
+            MSP_PFOR(idx, n << cs)
             {
-                const int col = idx >> sh;
+                const int col = idx >> cs;
                 const int lo = W.blo[col], hi = W.bhi[col];
-                const int row = lo + (idx & ((1 << sh) - 1));
-                if (row < hi) {
-                    if (hi - lo >= 2) {
-                        const int src = W.outsrc[col];
-                        if (src < 0) {
-                            const int c = W.perm[~src];
-                            const int sd = (row < lo + (hi - lo) / 2) ? 1 : 2;
-                            Qn[(size_t)col * n + row] = (W.sup[c] & sd) ? Q[(size_t)c * n + row] : 0.0;
+                const int r0 = lo + (idx & ((1 << cs) - 1));
+                if (hi - lo >= 2) {
+                    const int src = W.outsrc[col];
+                    if (src < 0) {
+                        const int c = W.perm[~src];
+                        const int mid = lo + (hi - lo) / 2;
+                        for (int row = r0; row < hi; row += 1 << cs) {
+                            const int sd = (row < mid) ? 1 : 2;
+                            Qn[(size_t)col * n + row] =
+                                (W.sup[c] & sd) ? Q[(size_t)c * n + row] : 0.0;
                         }
-                    } else {
-                        Qn[(size_t)col * n + row] = Q[(size_t)col * n + row];
                     }
+                } else {
+                    for (int row = r0; row < hi; row += 1 << cs)
+                        Qn[(size_t)col * n + row] = Q[(size_t)col * n + row];
                 }
             }
         }

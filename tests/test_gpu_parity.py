@@ -257,6 +257,25 @@ def test_muon_mixing_upper_block_gemm(eng, dims, mu):
             assert np.abs(g[b][mask] - npref[mask]).max() <= 1e-12 * f * np.abs(vec[b]).sum()
 
 
+def test_muon_mixing_gemm_rejects_other_multiplicity(eng):
+    """A system whose muon position holds a dimension other than 2 gets NaN
+    blocks from msp_gemm_muon_upper (the mix is defined for spin 1/2 only)."""
+    from muspinsim_amd import _native as N
+    dims, n, nb = (3, 4), 12, 2
+    sysarr = np.zeros(1, dtype=N.SYSTEM_DTYPE)
+    sysarr[0]["n_spins"] = 2
+    sysarr[0]["n"] = n
+    sysarr[0]["muon_index"] = 0
+    sysarr[0]["dims"][:2] = dims
+    dsys = eng._bytes_view(sysarr)
+    U = _dev(np.stack([np.eye(n)] * nb).astype(complex))
+    dv = _dev(np.ones((nb, 3)), torch.float64)
+    O = torch.zeros_like(U)
+    N.check(eng.lib.msp_gemm_muon_upper(n, nb, dsys.data_ptr(), None, dv.data_ptr(), 1,
+                                        U.data_ptr(), O.data_ptr(), eng.stream()))
+    assert torch.isnan(O.cpu()).all()
+
+
 def test_timesum_and_integral(eng):
     rng = np.random.default_rng(3)
     for n, T in ((4, 7), (16, 130), (70, 65)):
