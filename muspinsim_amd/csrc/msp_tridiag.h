@@ -182,6 +182,18 @@ MSP_HD void dc_block(int n, int L, int i, int &lo, int &hi)
     }
 }
 
+// Block number bi (left to right) of tree level L, for blocks whose
+// ancestors all have two or more positions.
+MSP_HD void dc_block_index(int n, int L, int bi, int &lo, int &hi)
+{
+    lo = 0;
+    hi = n;
+    for (int l = 0; l < L && hi - lo > 1; ++l) {
+        const int mid = lo + (hi - lo) / 2;
+        if ((bi >> (L - 1 - l)) & 1) lo = mid; else hi = mid;
+    }
+}
+
 // Largest block at tree level L (the rightmost path takes the ceiling half).
 MSP_HD int dc_maxblock(int n, int L)
 {
@@ -303,10 +315,15 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
 // col * n + row).  A kept column is nonzero only on its support, so a row
 // of the first child sums the product indices k < nka and a row of the
 // second k >= kb0 (kperm orders them); without rotations that halves the
-// work.  Large blocks (>= 48) run on FP64 matrix cores as Qn^T = S^T Q_K^T,
+// work.  Large blocks (>= kMergeMfmaMin) run on FP64 matrix cores as Qn^T = S^T Q_K^T,
 // each wave computing 32 x 32 output tiles from operands read in place,
 // every operand and result access along a contiguous 16-run; small ones are
 // direct dots.
+#ifndef MSP_MERGE_MFMA_MIN
+#define MSP_MERGE_MFMA_MIN 12
+#endif
+constexpr int kMergeMfmaMin = MSP_MERGE_MFMA_MIN;   // smallest merge block on MFMA
+
 MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double *Qn, double *red)
 {
     // small blocks: one output per iteration, (column, row in its block) items
@@ -318,7 +335,7 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
         const int col = idx >> sh;
         const int lo = W.blo[col], hi = W.bhi[col];
         const int row = lo + (idx & ((1 << sh) - 1));
-        if (hi - lo >= 2 && hi - lo < 48 && row < hi) {
+        if (hi - lo >= 2 && hi - lo < kMergeMfmaMin && row < hi) {
             int K = W.nkept[lo];
             int j = col - lo;
             if (j < K) {
@@ -333,14 +350,14 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
             }
         }
     }
-    // large blocks, one after another
+    // large blocks
+#ifdef MSP_HOST_BUILD
+    (void)red;
     for (int lo = 0; lo < n;) {
         const int blo = W.blo[lo], bhi = W.bhi[lo];
         const int sz = bhi - blo;
-        if (sz >= 48) {
+        if (sz >= kMergeMfmaMin) {
             const int K = W.nkept[blo];
-#ifdef MSP_HOST_BUILD
-            (void)red;
             for (int r = 0; r < sz; ++r) {
                 const bool first = r < sz / 2;
                 const int kb = first ? 0 : W.kb0[blo], ke = first ? W.nka[blo] : K;
@@ -353,66 +370,78 @@ MSP_HD void dc_merge_gemm(int n, int L, const DcWork &W, const double *Q, double
                     Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc;
                 }
             }
-#else
-            // 32 x 32 tiles of Qn^T = S^T Q_K^T (2 x 2 MFMA tiles of 16 x 16)
-            // per wave task: rows j = kept roots, columns r = block rows.
-            // A operand S^T[j][k] = S[k][j] (16 consecutive j of a row of S),
-            // B operand Q_K^T[k][r] = Q(blo + r, kcol[k]) (16 consecutive r of
-            // an eigenvector column); no LDS staging and no barriers
-            const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
-            const int li = lane & 15, lq = lane >> 4;
-            const int tj = (K + 31) / 32, tr = (sz + 31) / 32;
-            const int midl = sz / 2, nka = W.nka[blo], kb0 = W.kb0[blo];
-            for (int task = wave; task < tj * tr; task += nwaves) {
-                const int jt = (task / tr) * 32, rt = (task % tr) * 32;
-                // product range of this row tile: one child's rows sum their
-                // supported columns; a tile across the middle sums all with a
-                // support test per element
-                const int rend = (rt + 32 < sz) ? rt + 32 : sz;
-                const bool allA = rend <= midl, allB = rt >= midl;
-                const int kb = allB ? kb0 : 0, ke = allA ? nka : K;
-                msp_d4 acc[2][2];
-                for (int x = 0; x < 2; ++x)
-                    for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
-                for (int k0 = kb; k0 < ke; k0 += 4) {
-                    const int kk = k0 + lq;
-                    const bool kok = kk < ke;
-                    const int ki = W.kperm[blo + (kok ? kk : ke - 1)];
-                    const int qc = W.kcol[blo + ki];
-                    const int sp = (allA || allB) ? 3 : W.sup[qc];
-                    const double *Qk = Q + (size_t)qc * n + blo;
-                    const double *Sk = W.S + (size_t)(blo + ki) * n + blo;
-                    double a[2], bv[2];
-#pragma unroll
-                    for (int x = 0; x < 2; ++x) {
-                        const int j = jt + 16 * x + li;
-                        const int jc = (j < K) ? j : K - 1;
-                        const double va = Sk[jc];
-                        a[x] = (j < K && kok) ? va : 0.0;
-                        const int r = rt + 16 * x + li;
-                        const int rc = (r < sz) ? r : sz - 1;
-                        const double vb = Qk[rc];
-                        const bool on = sp & ((r < midl) ? 1 : 2);
-                        bv[x] = (r < sz && kok && on) ? vb : 0.0;
-                    }
-#pragma unroll
-                    for (int x = 0; x < 2; ++x)
-#pragma unroll
-                        for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = mfma_f64(a[x], bv[y2], acc[x][y2]);
-                }
-                for (int x = 0; x < 2; ++x)
-                    for (int y2 = 0; y2 < 2; ++y2)
-                        for (int q = 0; q < 4; ++q) {
-                            const int j = jt + 16 * x + 4 * q + lq;
-                            const int r = rt + 16 * y2 + li;
-                            if (j < K && r < sz) Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc[x][y2][q];
-                        }
-            }
-            (void)red;
-#endif
         }
         lo = (bhi > lo) ? bhi : lo + 1;
     }
+#else
+    {
+        // 32 x 32 tiles of Qn^T = S^T Q_K^T (2 x 2 MFMA tiles of 16 x 16)
+        // per wave task: rows j = kept roots, columns r = block rows.
+        // A operand S^T[j][k] = S[k][j] (16 consecutive j of a row of S),
+        // B operand Q_K^T[k][r] = Q(blo + r, kcol[k]) (16 consecutive r of
+        // an eigenvector column); no LDS staging and no barriers
+        const int lane = MSP_TID & 63, wave = MSP_TID >> 6, nwaves = MSP_NT >> 6;
+        const int li = lane & 15, lq = lane >> 4;
+        // the tasks of every large block of the level share the waves
+        const int tside = (smax + 31) / 32, tmax = tside * tside;
+        const int ntask = (smax >= kMergeMfmaMin) ? (1 << L) * tmax : 0;
+        for (int task = wave; task < ntask; task += nwaves) {
+            const int bi = task / tmax, tt = task - bi * tmax;
+            int blo, bhi;
+            dc_block_index(n, L, bi, blo, bhi);
+            const int sz = bhi - blo;
+            if (sz < kMergeMfmaMin) continue;
+            const int K = W.nkept[blo];
+            const int tj = (K + 31) / 32, tr = (sz + 31) / 32;
+            if (tt >= tj * tr) continue;
+            const int midl = sz / 2, nka = W.nka[blo], kb0 = W.kb0[blo];
+            const int jt = (tt / tr) * 32, rt = (tt % tr) * 32;
+            // product range of this row tile: one child's rows sum their
+            // supported columns; a tile across the middle sums all with a
+            // support test per element
+            const int rend = (rt + 32 < sz) ? rt + 32 : sz;
+            const bool allA = rend <= midl, allB = rt >= midl;
+            const int kb = allB ? kb0 : 0, ke = allA ? nka : K;
+            msp_d4 acc[2][2];
+            for (int x = 0; x < 2; ++x)
+                for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = d4zero();
+            for (int k0 = kb; k0 < ke; k0 += 4) {
+                const int kk = k0 + lq;
+                const bool kok = kk < ke;
+                const int ki = W.kperm[blo + (kok ? kk : ke - 1)];
+                const int qc = W.kcol[blo + ki];
+                const int sp = (allA || allB) ? 3 : W.sup[qc];
+                const double *Qk = Q + (size_t)qc * n + blo;
+                const double *Sk = W.S + (size_t)(blo + ki) * n + blo;
+                double a[2], bv[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    const int j = jt + 16 * x + li;
+                    const int jc = (j < K) ? j : K - 1;
+                    const double va = Sk[jc];
+                    a[x] = (j < K && kok) ? va : 0.0;
+                    const int r = rt + 16 * x + li;
+                    const int rc = (r < sz) ? r : sz - 1;
+                    const double vb = Qk[rc];
+                    const bool on = sp & ((r < midl) ? 1 : 2);
+                    bv[x] = (r < sz && kok && on) ? vb : 0.0;
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y2 = 0; y2 < 2; ++y2) acc[x][y2] = mfma_f64(a[x], bv[y2], acc[x][y2]);
+            }
+            for (int x = 0; x < 2; ++x)
+                for (int y2 = 0; y2 < 2; ++y2)
+                    for (int q = 0; q < 4; ++q) {
+                        const int j = jt + 16 * x + 4 * q + lq;
+                        const int r = rt + 16 * y2 + li;
+                        if (j < K && r < sz) Qn[(size_t)W.rootpos[blo + j] * n + blo + r] = acc[x][y2][q];
+                    }
+        }
+        (void)red;
+    }
+#endif
     MSP_SYNC();
 }
 
