@@ -815,12 +815,23 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                 if (hi - lo >= 2) {
                     const int src = W.outsrc[col];
                     if (src < 0) {
+                        // four rows per lane in flight: loads first, then stores
                         const int c = W.perm[~src];
-                        const int mid = lo + (hi - lo) / 2;
-                        for (int row = r0; row < hi; row += 1 << cs) {
-                            const int sd = (row < mid) ? 1 : 2;
-                            Qn[(size_t)col * n + row] =
-                                (W.sup[c] & sd) ? Q[(size_t)c * n + row] : 0.0;
+                        const int mid = lo + (hi - lo) / 2, sp = W.sup[c];
+                        for (int rb = r0; rb < hi; rb += 4 << cs) {
+                            double v[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const int row = rb + (u << cs);
+                                const int rc = (row < hi) ? row : hi - 1;
+                                const int sd = (rc < mid) ? 1 : 2;
+                                v[u] = (sp & sd) ? Q[(size_t)c * n + rc] : 0.0;
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const int row = rb + (u << cs);
+                                if (row < hi) Qn[(size_t)col * n + row] = v[u];
+                            }
                         }
                     }
                 } else {
