@@ -532,6 +532,9 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     zmax = fmax(zmax, fabs(W.zs[s]));
                 }
                 W.item_v[lo] = 8.0 * eps * fmax(dmax, zmax);
+                W.item_src[lo] = hi;      // first rotation slot: none yet
+                W.nkept[lo] = 0;
+                W.nrot[lo] = 0;
             }
         }
         MSP_SYNC();
@@ -564,32 +567,57 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
             }
         }
         MSP_SYNC();
-        // --- deflation scan, one thread per block: the slots before the
-        // first rotation take their precomputed flags; the scan tests
-        // (and rotates) from there on ---
+        // --- per slot, up to the block's first rotation: z-deflated slots
+        // to dfl, kept ones to dl/zl/kcol, at the positions the scan gives
+        // them (counts of the deflated / undeflated slots before); the slot
+        // that pairs with the first rotation leaves the scan state in
+        // item_src[lo] (start), outsrc[lo] (K), rootpos[lo] (nd) and
+        // kperm[lo] (pj), all rewritten later in the level ---
+        MSP_PFOR(s, n)
+        {
+            const int lo = W.blo[s], hi = W.bhi[s];
+            if (hi - lo >= 2) {
+                int nz = 0, nk = 0;
+                bool clean = true;
+                for (int t = lo; t < s; ++t) {
+                    const int dt = W.defl[t];
+                    clean = clean && (dt || !W.korg[t]);
+                    nz += dt;
+                    nk += 1 - dt;
+                }
+                if (clean && (W.defl[s] || !W.korg[s])) {
+                    if (W.defl[s]) {
+                        W.dfl[lo + nz] = s;
+                    } else {
+                        int t = s + 1;
+                        while (t < hi && W.defl[t]) ++t;
+                        if (t == hi || !W.korg[t]) {
+                            W.dl[lo + nk] = W.ds[s];
+                            W.zl[lo + nk] = W.zs[s];
+                            W.kcol[lo + nk] = W.perm[s];
+                            if (t == hi) W.nkept[lo] = nk + 1;
+                        } else {
+                            W.item_src[lo] = t;
+                            W.outsrc[lo] = nk;
+                            W.rootpos[lo] = nz + (t - s - 1);
+                            W.kperm[lo] = s;
+                        }
+                    }
+                }
+            }
+        }
+        MSP_SYNC();
+        // --- deflation scan from the first rotation on, one thread per
+        // block (blocks without a rotation are complete) ---
         MSP_PFOR(lo0, n)
         {
             const int lo = W.blo[lo0], hi = W.bhi[lo0];
-            if (lo == lo0 && hi - lo >= 2) {
+            if (lo == lo0 && hi - lo >= 2 && W.item_src[lo] < hi) {
                 int mid = lo + (hi - lo) / 2;
                 double rho = 2.0 * fabs(e[mid - 1] * inv);
                 const double tol = W.item_v[lo];
-                int K = 0, nr = 0, nd = 0, pj = -1;
-                int s0 = lo;
-                for (; s0 < hi; ++s0) {
-                    if (W.defl[s0]) {
-                        W.dfl[lo + nd++] = s0;
-                        continue;
-                    }
-                    if (W.korg[s0]) break;
-                    if (pj >= 0) {
-                        W.dl[lo + K] = W.ds[pj];
-                        W.zl[lo + K] = W.zs[pj];
-                        W.kcol[lo + K] = W.perm[pj];
-                        ++K;
-                    }
-                    pj = s0;
-                }
+                const int s0 = W.item_src[lo];
+                int K = W.outsrc[lo], nr = 0, nd = W.rootpos[lo], pj = W.kperm[lo];
                 for (int s = s0; s < hi; ++s) {
                     W.defl[s] = 0;
                     if (rho * fabs(W.zs[s]) <= tol) {
