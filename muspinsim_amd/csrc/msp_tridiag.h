@@ -234,6 +234,9 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
     if (interior) {
         double gap = dl[j + 1] - dl[j];
         double mid = 0.5 * gap;
+        // (unrolled: the reciprocal chains of four poles overlap; the sums
+        // keep their order)
+#pragma unroll 4
         for (int i = 0; i < K; ++i) {
             const double r = msp_recip((dl[i] - dl[j]) - mid);
             const double t = zz[i] * r;
@@ -268,6 +271,7 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
             phi = m_phi;
             dphi = m_dphi;
         } else {
+#pragma unroll 4
             for (int i = 0; i < K; ++i) {
                 const double r = msp_recip((dl[i] - dorg) - tau);
                 const double t = zz[i] * r;
@@ -747,6 +751,7 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     const double *dl = W.dl + lo;
                     double di = dl[i];
                     double wv = 1.0;
+#pragma unroll 4
                     for (int j = 0; j < K; ++j) {
                         int org = W.korg[lo + j];
                         double del = (di - dl[org]) - W.ktau[lo + j];  // d_i - lambda_j
@@ -773,11 +778,13 @@ MSP_HD double *stedc_team(int n, double *d, const double *e, DcWork W, double *r
                     // the norm first, then each entry once (recomputed, not
                     // read back)
                     double nrm = 0.0;
+#pragma unroll 4
                     for (int i = 0; i < K; ++i) {
                         const double u = W.zh[lo + i] * msp_recip((dl[i] - dl[org]) - tj);
                         nrm += u * u;
                     }
                     nrm = 1.0 / sqrt(nrm);
+#pragma unroll 4
                     for (int i = 0; i < K; ++i)
                         W.S[(size_t)(lo + i) * n + (lo + j)] =
                             W.zh[lo + i] * nrm * msp_recip((dl[i] - dl[org]) - tj);
