@@ -45,8 +45,9 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
                                                                double *U)
 {
     __shared__ double Wp[2][NW][2][256];      // per-wave partial W (re, im), by block parity
-    const int b = blockIdx.y;
-    const int c0 = blockIdx.x * CW;
+    int chunk, b;
+    msp_xcd_tile(gridDim.x, gridDim.y, chunk, b);
+    const int c0 = chunk * CW;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lq = lane >> 4;
     const int ntiles = (n + 15) / 16;

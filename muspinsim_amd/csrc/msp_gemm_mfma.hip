@@ -27,8 +27,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSystem *systems,
                const int32_t *sys_id, const double *vec, int pauli)
 {
-    if (UPPER && blockIdx.y > blockIdx.x) return;
-    const int b = blockIdx.z;
+    int tile, b;
+    msp_xcd_tile(gridDim.x * gridDim.y, gridDim.z, tile, b);
+    const int bx = tile % gridDim.x, by = tile / gridDim.x;
+    if (UPPER && by > bx) return;
     int mstride = 1;
     bool mix_ok = true;
     cdbl m_same[2], m_other[2];
@@ -46,8 +48,8 @@ k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSyst
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
-    const int i0 = blockIdx.y * 64 + 32 * (wave >> 1);
-    const int j0 = blockIdx.x * 64 + 32 * (wave & 1);
+    const int i0 = by * 64 + 32 * (wave >> 1);
+    const int j0 = bx * 64 + 32 * (wave & 1);
     const cdbl *Ab = (const cdbl *)A + (size_t)b * n * n;
     const cdbl *Bb = (const cdbl *)B + (size_t)b * n * n;
     msp_d4 cr[2][2], ci[2][2];

@@ -30,3 +30,24 @@ __device__ inline msp_d4 d4zero()
     msp_d4 z = {0.0, 0.0, 0.0, 0.0};
     return z;
 }
+
+// XCD-aware tile order for grids of `ntile` tiles per instance.  Workgroups
+// go to the eight XCDs round-robin by linear id L (x fastest), so XCD L % 8
+// receives L, L + 8, L + 16, ...; those are handed the tiles of one instance
+// in turn, and the tiles of an instance run on one XCD at about the same
+// time and share its L2 (operands read by every tile: U, the V T blocks).
+// The last batch % 8 instances keep the plain order.  Any order is correct;
+// only the cache reuse depends on it.
+__device__ inline void msp_xcd_tile(int ntile, int batch, int &tile, int &inst)
+{
+    const int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int full = (batch / 8) * 8 * ntile;
+    if (L < full) {
+        const int slot = L >> 3;
+        inst = (slot / ntile) * 8 + (L & 7);
+        tile = slot - (slot / ntile) * ntile;
+    } else {
+        inst = L / ntile;
+        tile = L - inst * ntile;
+    }
+}
