@@ -115,8 +115,9 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 8}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(instances shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (instances "
+              "are independent and would shard by contiguous ranges, one process per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "celio":
