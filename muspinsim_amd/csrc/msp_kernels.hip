@@ -620,8 +620,9 @@ __global__ void __launch_bounds__(256) k_timesum(int n, int ntimes, const double
     __shared__ double Cs_im[REAL ? 1 : TI][TJ + 1];
     __shared__ cdbl Ph[TJ][TT + 1];
     __shared__ cdbl red[16][TT + 1];
-    const int b = blockIdx.y;
-    const int t0 = blockIdx.x * TT;
+    int tt, b;
+    msp_xcd_tile(gridDim.x, gridDim.y, tt, b);   // an instance's time tiles share one L2
+    const int t0 = tt * TT;
     const int tg = threadIdx.x % 16, ig = threadIdx.x / 16;
     const double *Eb = E + (size_t)b * n;
     const double Ec = 0.5 * (Eb[0] + Eb[n - 1]);
