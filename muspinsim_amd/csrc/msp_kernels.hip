@@ -159,7 +159,7 @@ __global__ void k_build_h(int n, const MspSystem *systems, const MspTerm *terms,
 // non-zero nibble whose digits differ by one means a one-spin flip (S_x, S_y),
 // anything else is the base element alone.  Systems with a multiplicity above
 // 16 take the per-element path.
-constexpr int kBuildEPT = 4;
+constexpr int kBuildEPT = 16;
 __global__ void __launch_bounds__(256) k_build_h_base(int n, const MspSystem *systems,
                                                       const MspTerm *terms, const int32_t *sys_id,
                                                       const double *Br, const double *base,
