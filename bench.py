@@ -96,7 +96,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-sample", type=int, default=24000,
                     help="instances of the sweep timed on the host (0 = skip)")
-    ap.add_argument("--cpu-procs", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="0 = the job's CPU share (affinity set, limited by OMP_NUM_THREADS)")
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
     ap.add_argument("--chunk-gib", type=float, default=0.0,
                     help="device memory budget of one chunk in GiB (0 = engine default)")
@@ -138,19 +139,26 @@ def main():
         from oracle import baseline
         stride = max(1, ninst // args.cpu_sample)
         sample = np.arange(0, ninst, stride)[: args.cpu_sample]
-        procs = min(args.cpu_procs or min(16, baseline.host_cores()), len(sample))
+        procs = min(args.cpu_procs or baseline.cpu_share(), len(sample))
         cpu_rows, wall, busy = baseline.run(systems, batch, sample, procs)
         one = sample[: max(1, min(len(sample), 64))]
         _, wall1, _ = baseline.run(systems, batch, one, 1)
+        host = baseline.host_info()
+        rate = len(sample) / wall
         cpu = {
-            "value": len(sample) / wall,
+            "value": rate,
             "unit": "orientation-solves/s",
             "cores": procs,
             "kind": "port",
             "sample": f"{len(sample)} of {ninst} instances (stride {stride}) of the same sweep, "
                       f"numpy FP64 eigh + basis change + integral, {procs} processes x 1 BLAS "
-                      f"thread on {baseline.cpu_model()}",
+                      f"thread (instances[r::P], summed) on {host['model']}",
             "single_core_value": len(one) / wall1,
+            "host": host,
+            # the reference's MPI split scales at best linearly in processes:
+            # the measured per-process rate times every CPU of the affinity
+            # set bounds what the whole host could reach
+            "full_host_linear_bound": rate / procs * host["affinity_cpus"],
         }
 
     # ---- GPU
@@ -253,6 +261,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "gpu_vs_cpu_full_host_bound": (value / cpu["full_host_linear_bound"]) if cpu else None,
         "results_checksum": float(np.sum(results)),
     }
     if args.workload == "c5":

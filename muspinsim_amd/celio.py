@@ -224,7 +224,10 @@ def _exp_factors(eng, Hs, signs, theta):
     if int(info.max().item()) != 0:
         raise np.linalg.LinAlgError("Eigenvalues did not converge")
     ph = torch.polar(torch.ones_like(E), signs[:, None] * theta * E)
-    return torch.matmul(U * ph[:, None, :], U.conj().transpose(-1, -2).contiguous())
+    # U D U^H = A^H B with A = (U D)^H and B = U^H, on msp_gemm_cn
+    A = (U * ph[:, None, :]).conj().transpose(-1, -2).contiguous()
+    B = U.conj().transpose(-1, -2).contiguous()
+    return eng.gemm_cn(A, B)
 
 
 class CelioJob:
@@ -394,7 +397,7 @@ class CelioJob:
             idx = torch.as_tensor(np.array([off for off, _, _ in items])[:, None] +
                                   np.arange(m * m)[None, :], device=dev)
             mats[idx.reshape(-1)] = C.reshape(-1)
-        self._compose(mats, dev)
+        self._compose(mats, dev, eng)
         tables = np.concatenate(self._tables).astype(np.int32)
         for (m, _, fib, el) in self._desc + self._meas:
             if fib + self.n // m > tables.size or el + m > tables.size:
@@ -417,9 +420,10 @@ class CelioJob:
         prep["wsb"] = int(wsb)
         return prep
 
-    def _compose(self, mats, dev):
+    def _compose(self, mats, dev, eng):
         """Products of the fused factors, batched by pattern (frame and part
-        spins): F = E_last ... E_1 with E_j the j-th factor on the frame."""
+        spins): F = E_last ... E_1 with E_j the j-th factor on the frame
+        (E F = (E^H)^H F on msp_gemm_cn)."""
         pats = {}
         for off, frame, parts in self._composites:
             key = (frame, tuple(tuple(sp) for _, sp, _ in parts))
@@ -434,7 +438,7 @@ class CelioJob:
                 E = torch.zeros((len(items), m * m), dtype=torch.complex128, device=dev)
                 E[:, torch.as_tensor(ia * m + ib, device=dev)] = mats[src]
                 E = E.reshape(len(items), m, m)
-                F = E if F is None else torch.matmul(E, F)
+                F = E if F is None else eng.gemm_cn(E.conj().transpose(-1, -2).contiguous(), F)
             dst = torch.as_tensor(np.array([off for off, _ in items])[:, None] +
                                   np.arange(m * m)[None, :], device=dev)
             mats[dst.reshape(-1)] = F.reshape(-1)

@@ -37,8 +37,11 @@ k_gemm_cn_mfma(int n, const double *A, const double *B, double *C, const MspSyst
     if constexpr (MIX) {
         const MspSystem &S = systems[sys_id ? sys_id[b] : 0];
         mix_ok = S.dims[S.muon_index] == 2;   // otherwise the block writes NaN
-        if (!mix_ok) mstride = 0x40000000;    // (keeps every partner row in range)
-        for (int k = S.n_spins - 1; k > S.muon_index; --k) mstride *= S.dims[k];
+        if (mix_ok) {
+            for (int k = S.n_spins - 1; k > S.muon_index; --k) mstride *= S.dims[k];
+        } else {
+            mstride = n;                      // (kc / n) & 1 == 0; the partner row is kc itself
+        }
         const double f = pauli ? 1.0 : 0.5;        // v.sigma or v.S
         const double vx = vec[3 * b], vy = vec[3 * b + 1], vz = vec[3 * b + 2];
         m_same[0] = cmk(f * vz, 0.0);              // M[0][0]

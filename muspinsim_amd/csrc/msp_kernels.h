@@ -36,8 +36,11 @@ hipError_t launch_build_h(int n, int batch, const MspSystem *systems, const MspT
                           int lower, double *H, hipStream_t s);
 hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, double *U,
                               int32_t *info, int max_sweeps, hipStream_t s);
+// overlap: form the WY factors on a per-thread side stream while the divide
+// and conquer runs (forked from and joined back into s); false keeps every
+// launch on s and creates no events.
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
-                          double *work, hipStream_t s);
+                          double *work, hipStream_t s, bool overlap = true);
 hipError_t launch_apply_muon(int n, int batch, const MspSystem *systems, const int32_t *sys_id,
                              const double *vec, int pauli, const double *U, double *Y,
                              hipStream_t s);

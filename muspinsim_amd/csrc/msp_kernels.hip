@@ -991,22 +991,61 @@ hipError_t msp_set_lds(const void *func, size_t bytes)
     return err;
 }
 
-// Second stream of each device, for work that overlaps the divide and conquer
-// (created on first use; default priority; nullptr if it cannot be made).
+// Second stream of each device and calling thread, for work that overlaps
+// the divide and conquer (created on first use; default priority; nullptr if
+// it cannot be made).  Per thread, so that one thread's msp_eigh never
+// enqueues onto a stream that another thread's graph capture has joined.
 static hipStream_t side_stream()
 {
-    static hipStream_t streams[64] = {};
-    static std::mutex mu;
+    thread_local hipStream_t streams[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    std::lock_guard<std::mutex> lock(mu);
     if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
         streams[dev] = nullptr;
     return streams[dev];
 }
 
+// The tridiagonal eigensolver on s and, when s2 is given, the WY factors on
+// s2 (forked from s after the tridiagonalisation, joined back before the
+// back-transformation); every exit after the fork joins s2 into s.
+static hipError_t eigh_dc_middle(int n, int batch, double *H, double *E, int32_t *info,
+                                 double *work, hipStream_t s, hipStream_t s2,
+                                 hipEvent_t ev_td, hipEvent_t ev_ft)
+{
+    hipError_t err = hipSuccess;
+    bool forked = false;
+    if (s2) {
+        if ((err = hipEventRecord(ev_td, s)) != hipSuccess) return err;
+        if ((err = hipStreamWaitEvent(s2, ev_td, 0)) != hipSuccess) return err;
+        forked = true;
+        ProfScope ps_("k_larft", s2);
+        err = launch_larft(n, batch, (const double *)H, work, s2);
+    }
+    if (err == hipSuccess) {
+        const int in_lds = (n <= 512) ? 1 : 0;
+        const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
+        err = msp_set_lds((const void *)k_stedc, lds2);
+        if (err == hipSuccess) {
+            ProfScope ps_("k_stedc", s);
+            hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
+            err = hipGetLastError();
+        }
+    }
+    if (forked) {
+        // join even after a failure, so that no capture ends with s2 unjoined
+        hipError_t e1 = hipEventRecord(ev_ft, s2);
+        hipError_t e2 = (e1 == hipSuccess) ? hipStreamWaitEvent(s, ev_ft, 0) : e1;
+        if (err == hipSuccess) err = e2;
+    }
+    if (err == hipSuccess && !s2 && bt_nblocks(n) > 0) {
+        ProfScope ps_("k_larft", s);
+        err = launch_larft(n, batch, (const double *)H, work, s);
+    }
+    return err;
+}
+
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
-                          double *work, hipStream_t s)
+                          double *work, hipStream_t s, bool overlap)
 {
     hipError_t err = hipSuccess;
     if (n <= 1024) {
@@ -1026,7 +1065,7 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     // stream while the divide and conquer runs (it depends on d and e only);
     // the back-transformation joins both.  Events are per call, so
     // concurrent callers on other streams keep their own dependencies.
-    hipStream_t s2 = (bt_nblocks(n) > 0) ? side_stream() : nullptr;
+    hipStream_t s2 = (overlap && bt_nblocks(n) > 0) ? side_stream() : nullptr;
     hipEvent_t ev_td = nullptr, ev_ft = nullptr;
     if (s2 && (hipEventCreateWithFlags(&ev_td, hipEventDisableTiming) != hipSuccess ||
                hipEventCreateWithFlags(&ev_ft, hipEventDisableTiming) != hipSuccess)) {
@@ -1034,34 +1073,10 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
         ev_td = nullptr;
         s2 = nullptr;
     }
-    if (s2) {
-        if ((err = hipEventRecord(ev_td, s)) != hipSuccess) return err;
-        if ((err = hipStreamWaitEvent(s2, ev_td, 0)) != hipSuccess) return err;
-        {
-            ProfScope ps_("k_larft", s2);
-            err = launch_larft(n, batch, (const double *)H, work, s2);
-        }
-        if (err != hipSuccess) return err;
-        if ((err = hipEventRecord(ev_ft, s2)) != hipSuccess) return err;
-    }
-    {
-        const int in_lds = (n <= 512) ? 1 : 0;
-        const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
-        err = msp_set_lds((const void *)k_stedc, lds2);
-        if (err != hipSuccess) return err;
-        ProfScope ps_("k_stedc", s);
-        hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
-    }
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    if (s2) {
-        err = hipStreamWaitEvent(s, ev_ft, 0);
-        hipEventDestroy(ev_td);
-        hipEventDestroy(ev_ft);
-        if (err != hipSuccess) return err;
-    } else if (bt_nblocks(n) > 0) {
-        ProfScope ps_("k_larft", s);
-        if ((err = launch_larft(n, batch, (const double *)H, work, s)) != hipSuccess) return err;
-    }
+    err = eigh_dc_middle(n, batch, H, E, info, work, s, s2, ev_td, ev_ft);
+    if (ev_td) hipEventDestroy(ev_td);
+    if (ev_ft) hipEventDestroy(ev_ft);
+    if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (n <= 1024) {
         ProfScope ps_("k_backtransform", s);

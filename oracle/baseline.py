@@ -61,7 +61,26 @@ def cpu_model():
 
 
 def host_cores():
+    """CPUs this process may run on (its affinity set)."""
     try:
         return len(os.sched_getaffinity(0))
     except AttributeError:
         return os.cpu_count() or 1
+
+
+def cpu_share():
+    """Worker processes the baseline may use: the affinity set, limited to
+    the job's CPU share where the host states one through OMP_NUM_THREADS
+    (a node that hosts one job per GPU gives each job such a share, and the
+    other cores belong to the other GPUs' jobs)."""
+    n = host_cores()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def host_info():
+    """CPU model and counts recorded beside the baseline."""
+    return {"model": cpu_model(), "logical_cpus": os.cpu_count(),
+            "affinity_cpus": host_cores(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}

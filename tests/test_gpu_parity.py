@@ -276,6 +276,69 @@ def test_muon_mixing_gemm_rejects_other_multiplicity(eng):
     assert torch.isnan(O.cpu()).all()
 
 
+def test_xcd_tile_order_batch11(eng):
+    """Batch 11 at n = 192: the first 8 instances take the XCD-grouped tile
+    order and the last 3 the plain tail order, in one launch each of
+    msp_gemm_cn_upper, msp_gemm_muon_upper, msp_timesum and the divide and
+    conquer eigh (MFMA back-transformation); all checked against numpy."""
+    from muspinsim_amd import _native as N
+    n, nb = 192, 11
+    rng = np.random.default_rng(1911)
+    Hs = np.stack([_herm(rng, n, 20.0) for _ in range(nb)])
+    E, U, info = eng.eigh(_dev(Hs), method=2)
+    E, Ud = E.cpu().numpy(), U
+    Un = U.cpu().numpy()
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(nb):
+        Er = np.linalg.eigvalsh(Hs[b])
+        nrm = np.abs(Er).max()
+        assert np.abs(E[b] - Er).max() <= 1e-13 * n * nrm
+        assert np.abs(Hs[b] @ Un[b] - Un[b] * E[b]).max() <= 1e-13 * n * nrm
+        assert np.abs(Un[b].conj().T @ Un[b] - np.eye(n)).max() <= 1e-13 * n
+    st = eng.stream()
+    blk = np.arange(n) // 64
+    mask = blk[:, None] <= blk[None, :]
+    # upper-block GEMM of U^H (M U)
+    M = np.stack([_herm(rng, n) for _ in range(nb)])
+    Y = _dev(M @ Un)
+    up = torch.full_like(Ud, 7.0)
+    N.check(eng.lib.msp_gemm_cn_upper(n, nb, Ud.data_ptr(), Y.data_ptr(), up.data_ptr(), st))
+    u = up.cpu().numpy()
+    for b in range(nb):
+        ref = Un[b].conj().T @ M[b] @ Un[b]
+        assert np.abs(u[b][mask] - ref[mask]).max() <= 1e-12 * np.abs(M[b]).max() * n ** 0.5
+        assert np.all(u[b][~mask] == 7.0)
+    # muon-mixing upper GEMM, muon first of (2, 96)
+    sysarr = np.zeros(1, dtype=N.SYSTEM_DTYPE)
+    sysarr[0]["n_spins"] = 2
+    sysarr[0]["n"] = n
+    sysarr[0]["muon_index"] = 0
+    sysarr[0]["dims"][:2] = (2, 96)
+    dsys = eng._bytes_view(sysarr)
+    vec = rng.normal(size=(nb, 3))
+    dv = _dev(vec, torch.float64)
+    got = torch.full_like(Ud, 7.0)
+    N.check(eng.lib.msp_gemm_muon_upper(n, nb, dsys.data_ptr(), None, dv.data_ptr(), 1,
+                                        Ud.data_ptr(), got.data_ptr(), st))
+    g = got.cpu().numpy()
+    sx = np.array([[0, 1], [1, 0]]); sy = np.array([[0, -1j], [1j, 0]]); sz = np.diag([1.0, -1])
+    for b in range(nb):
+        op = np.kron(vec[b, 0] * sx + vec[b, 1] * sy + vec[b, 2] * sz, np.eye(96))
+        ref = Un[b].conj().T @ op @ Un[b]
+        assert np.abs(g[b][mask] - ref[mask]).max() <= 1e-12 * np.abs(vec[b]).sum()
+        assert np.all(g[b][~mask] == 7.0)
+    # time sum (complex C), 3 time tiles
+    T = 150
+    C = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    t = np.linspace(0, 10, T)
+    out = eng.timesum(_dev(E, torch.float64), _dev(C), _dev(t, torch.float64), False,
+                      None, True).cpu().numpy()
+    for b in range(nb):
+        ph = np.exp(-2j * np.pi * E[b][None, :] * t[:, None])           # (T, n)
+        ref = np.einsum("ti,ij,tj->t", ph, C[b], ph.conj())
+        assert np.abs(out[b] - ref).max() <= 1e-12 * np.abs(C[b]).sum()
+
+
 def test_timesum_and_integral(eng):
     rng = np.random.default_rng(3)
     for n, T in ((4, 7), (16, 130), (70, 65)):
@@ -345,6 +408,33 @@ def test_c4_small_buckets_parity(eng):
                         T=batch.T[keep], w=batch.w[keep], slot=batch.slot[keep],
                         sys_id=batch.sys_id[keep], times=batch.times, y=batch.y,
                         n_slots=1, avg_N=len(keep))
+    _end_to_end(systems, sub, eng)
+
+
+def _c4_subset(picks):
+    from muspinsim_amd import workloads as W
+    from muspinsim_amd.engine import InstanceBatch
+    systems, batch = W.c4_uneven_batch()
+    keep = np.array(picks)
+    sub = InstanceBatch(q=batch.q[keep], B=batch.B[keep], Bint=batch.Bint[keep], p=batch.p[keep],
+                        T=batch.T[keep], w=batch.w[keep], slot=batch.slot[keep],
+                        sys_id=batch.sys_id[keep], times=batch.times, y=batch.y,
+                        n_slots=1, avg_N=len(keep))
+    return systems, sub
+
+
+def test_c4_large_buckets_parity(eng):
+    """Uneven batch, end to end through H build -> eigh -> basis change ->
+    time sum: two instances of the full 4096-instance batch from each of
+    the n = 384, 512, 768 and 1024 buckets (panel pair in the workspace,
+    8-wave MFMA back-transformation above n = 512), all in one call."""
+    from muspinsim_amd import workloads as W
+    systems, _ = W.c4_uneven_batch()
+    ns = np.array([s.n for s in systems])
+    picks = []
+    for n in (384, 512, 768, 1024):
+        picks += list(np.nonzero(ns == n)[0][:2])
+    systems, sub = _c4_subset(picks)
     _end_to_end(systems, sub, eng)
 
 

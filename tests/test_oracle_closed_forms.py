@@ -263,3 +263,84 @@ def test_get_xy():
     assert np.allclose(np.cross(x, y), z / np.linalg.norm(z))
     with pytest.raises(ValueError):
         physics.get_xy([0, 0, 0])
+
+
+def _dipolar_modules():
+    from muspinsim_amd import constants as product_constants
+    return [physics, product_constants]
+
+
+@pytest.mark.parametrize("mod", _dipolar_modules(), ids=["oracle", "package"])
+def test_dipolar_tensor_textbook(mod):
+    """Dipolar tensor (spinsys.py:400-409) against the textbook form
+    H_D = b [S_i.S_j - 3 (S_i.r)(S_j.r)], b = (mu0 / 4 pi) hbar g_i g_j / r^3 with
+    the gyromagnetic ratios in rad s^-1 T^-1 and b converted from rad/s to MHz.
+    For two protons 1 A apart b is the familiar 120.1 kHz.  The tensor is
+    b (1 - 3 r r^T): traceless, eigenvalue -2b along r and +b twice across it."""
+    gH = 42.577478518                       # MHz/T, pinned by tests/test_constants.py
+    grad = 2 * np.pi * gH * 1e6             # rad s^-1 T^-1
+
+    def b_mhz(g1, g2, r_ang):
+        b = cnst.mu_0 / (4 * np.pi) * cnst.hbar * (2 * np.pi * g1 * 1e6) * (2 * np.pi * g2 * 1e6)
+        return b / (r_ang * 1e-10) ** 3 / (2 * np.pi) / 1e6
+
+    b1 = cnst.mu_0 / (4 * np.pi) * cnst.hbar * grad ** 2 / (1e-10) ** 3 / (2 * np.pi) / 1e6
+    assert abs(b1 - 0.1201) < 5e-5
+    rng = np.random.default_rng(17)
+    pairs = [(gH, gH), (physics.MU_GAMMA, gH), (physics.ELEC_GAMMA, physics.MU_GAMMA)]
+    for g1, g2 in pairs:
+        for _ in range(3):
+            r = rng.normal(size=3)
+            r *= rng.uniform(0.8, 3.0) / np.linalg.norm(r)
+            rn = np.linalg.norm(r)
+            rh = r / rn
+            b = b_mhz(g1, g2, rn)
+            D = mod.dipolar_tensor(g1, g2, r)
+            assert np.allclose(D, b * (np.eye(3) - 3 * np.outer(rh, rh)), rtol=1e-12, atol=0)
+            assert abs(np.trace(D)) <= 1e-12 * abs(b)
+            assert np.allclose(D @ rh, -2 * b * rh, rtol=1e-12, atol=1e-15 * abs(b))
+            ev = np.sort(np.linalg.eigvalsh(D))
+            assert np.allclose(ev, np.sort([-2 * b, b, b]), rtol=1e-12, atol=1e-15 * abs(b))
+    # the operator: two protons on the z axis, H = b (S1.S2 - 3 S1z S2z)
+    D = mod.dipolar_tensor(gH, gH, [0, 0, 1.5])
+    H = physics.hamiltonian([0.5, 0.5], [("bilinear", 0, 1, D)])
+    s = [spin.single(0.5, a) for a in "xyz"]
+    SS = sum(np.kron(x, x) for x in s)
+    b = b_mhz(gH, gH, 1.5)
+    assert np.allclose(H, b * (SS - 3 * np.kron(s[2], s[2])), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("mod", _dipolar_modules(), ids=["oracle", "package"])
+def test_quadrupolar_spin1_closed_form(mod):
+    """Quadrupolar tensor (spinsys.py:433-438) through the spin-1 closed form.
+    EFG_2_MHZ is rebuilt from CODATA as e (E_h / (e a0^2)) (1 mb) / h in MHz.
+    For an EFG with principal values |Vzz| >= |Vyy| >= |Vxx| (atomic units) in
+    a random frame and eta = (Vxx - Vyy) / Vzz, H_Q = sum_ab Q_ab S_a S_b has
+    eigenvalues (Cq/4)(1 - eta), (Cq/4)(1 + eta) and -Cq/2 with
+    Cq = EFG_2_MHZ Q Vzz; the spin-3/2 levels are +-(Cq/4) sqrt(1 + eta^2/3),
+    each twice (Kramers pairs)."""
+    Eh = cnst.physical_constants["Hartree energy"][0]
+    a0 = cnst.physical_constants["Bohr radius"][0]
+    efg_au = Eh / (cnst.e * a0 ** 2)                     # V / m^2
+    k = cnst.e * efg_au * 1e-31 / cnst.h / 1e6           # MHz per (a.u. x mb)
+    assert np.isclose(mod.EFG_2_MHZ, k, rtol=1e-8, atol=0)
+    rng = np.random.default_rng(23)
+    for _ in range(4):
+        Vzz = rng.uniform(0.1, 2.0) * rng.choice([-1, 1])
+        eta = rng.uniform(0, 1)
+        Vxx = -Vzz * (1 - eta) / 2
+        Vyy = -Vzz * (1 + eta) / 2
+        Rq, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        V = Rq @ np.diag([Vxx, Vyy, Vzz]) @ Rq.T
+        Q = rng.uniform(1.0, 50.0)
+        Cq = k * Q * Vzz
+        H = physics.hamiltonian([1.0], [("bilinear", 0, 0, mod.quadrupolar_tensor(Q, 1.0, V))])
+        ev = np.sort(np.linalg.eigvalsh(H))
+        ref = np.sort([Cq / 4 * (1 - eta), Cq / 4 * (1 + eta), -Cq / 2])
+        assert np.allclose(ev, ref, rtol=1e-10, atol=1e-12 * abs(Cq))
+        H = physics.hamiltonian([1.5], [("bilinear", 0, 0, mod.quadrupolar_tensor(Q, 1.5, V))])
+        ev = np.sort(np.linalg.eigvalsh(H))
+        e0 = abs(Cq) / 4 * np.sqrt(1 + eta ** 2 / 3)
+        assert np.allclose(ev, [-e0, -e0, e0, e0], rtol=1e-10, atol=1e-12 * abs(Cq))
+    with pytest.raises(ValueError):
+        mod.quadrupolar_tensor(1.0, 0.5, np.eye(3))

@@ -130,13 +130,36 @@ def test_pipeline_empty_and_precession():
 
 def test_pipeline_alc_integral():
     """tests/test_experiment.py:153-202: integral = 0.5 / (1 + 4 pi^2 tau^2)
-    for field and intrinsic field along z, transverse polarisation."""
+    for a 1 MHz local muon Zeeman term, transverse polarisation, on both the
+    field and the intrinsic-field axis.  A field b along z adds gamma_mu b to
+    the precession frequency, so the integral is 0.5 / (1 + 4 pi^2 tau^2 f^2)
+    with f = 1 + gamma_mu b: an applied field reaches the muon rotated into
+    the crystallite frame (experiment.py:405-406), an intrinsic one is not
+    rotated (experiment.py:407), so under a tilted orientation only the
+    intrinsic field keeps the closed form."""
     ge, gmu = physics.ELEC_GAMMA, physics.MU_GAMMA
     tau = physics.MU_TAU
     s = _sys([0.5, 0.5], [ge, gmu], [("linear", 1, 1, np.array([0, 0, 1.0 / gmu]) * gmu)], 1)
-    for B, Bint in [([0, 0, 0], [0, 0, 0]), ([0, 0, 0], [0, 0, 0])]:
-        res = pipeline.run([s], _batch("integral", [np.inf], B, [1, 0, 0], np.inf, Bint=Bint))
-        assert np.isclose(res[0, 0], 0.5 / (1.0 + 4 * np.pi ** 2 * tau ** 2))
+
+    def closed(b):
+        f = 1.0 + gmu * b
+        return 0.5 / (1.0 + 4 * np.pi ** 2 * tau ** 2 * f ** 2)
+
+    ident = [1.0, 0, 0, 0]
+    tilt = np.array([np.cos(0.35), np.sin(0.35), 0.0, 0.0])      # rotation about x
+    for b in (0.0, 0.004, -0.011, 0.05):
+        bz = [0, 0, b]
+        res = pipeline.run([s], _batch("integral", [np.inf], bz, [1, 0, 0], np.inf, q=ident))
+        assert np.isclose(res[0, 0], closed(b), rtol=1e-12, atol=0)
+        for q in (ident, tilt):
+            res = pipeline.run([s], _batch("integral", [np.inf], [0, 0, 0], [1, 0, 0], np.inf,
+                                           q=q, Bint=bz))
+            # the polarisation is rotated with the crystal too; along x it is
+            # still transverse to z for a rotation about x
+            assert np.isclose(res[0, 0], closed(b), rtol=1e-12, atol=0)
+        if b != 0.0:
+            res = pipeline.run([s], _batch("integral", [np.inf], bz, [1, 0, 0], np.inf, q=tilt))
+            assert not np.isclose(res[0, 0], closed(b), rtol=1e-6, atol=0)
 
 
 def test_pipeline_dissipation():
