@@ -113,6 +113,14 @@ def rotate_host(q, v):
     return v + 2.0 * (w * c + np.cross(u, c))
 
 
+def _rowmajor(*ts):
+    """The tensors as row-major (contiguous) device buffers: the C ABI reads
+    matrices through data_ptr() as dense row-major arrays, so a strided view
+    (a transpose, a slice) is copied first.  None passes through."""
+    out = tuple(t if t is None or t.is_contiguous() else t.contiguous() for t in ts)
+    return out if len(out) > 1 else out[0]
+
+
 class SpinEngine:
     """Runs instance batches through the HIP pipeline on one device."""
 
@@ -162,6 +170,7 @@ class SpinEngine:
     def eigh(self, H, method=None):
         """E (B, n) ascending and U (B, n, n) of a batch of Hermitian matrices
         (complex128, destroyed)."""
+        H = _rowmajor(H)
         B, n, _ = H.shape
         m = self.method if method is None else method
         E = torch.empty((B, n), dtype=torch.float64, device=self.device)
@@ -174,12 +183,14 @@ class SpinEngine:
         return E, U, info
 
     def gemm_cn(self, A, Bm):
+        A, Bm = _rowmajor(A, Bm)
         C = torch.empty_like(A)
         N.check(self.lib.msp_gemm_cn(A.shape[-1], A.shape[0], A.data_ptr(), Bm.data_ptr(),
                                      C.data_ptr(), self.stream()))
         return C
 
     def form_c(self, rho, obs, real_mode, alpha=0.0, beta=0.0):
+        rho, obs = _rowmajor(rho, obs)
         B, n, _ = obs.shape
         if real_mode:
             C = torch.empty((B, n, n), dtype=torch.float64, device=self.device)
@@ -191,6 +202,7 @@ class SpinEngine:
         return C
 
     def timesum(self, E, C, times, real_mode, w=None, out_complex=False):
+        E, C, times, w = _rowmajor(E, C, times, w)
         B, n = E.shape
         T = times.shape[0]
         if out_complex:
@@ -203,6 +215,7 @@ class SpinEngine:
         return out
 
     def integral(self, E, C, real_mode, tau, out_scale=1.0, w=None, out_complex=False):
+        E, C, w = _rowmajor(E, C, w)
         B, n = E.shape
         if out_complex:
             out = torch.empty((B,), dtype=torch.complex128, device=self.device)
@@ -215,6 +228,7 @@ class SpinEngine:
 
     def phase_evolve(self, E, rho, times):
         """(T, n, n): exp(-2 pi i (E_i - E_j) t) rho_ij for one instance."""
+        E, rho, times = _rowmajor(E, rho, times)
         n = E.shape[-1]
         T = times.shape[0]
         out = torch.empty((T, n, n), dtype=torch.complex128, device=self.device)
@@ -223,6 +237,7 @@ class SpinEngine:
         return out
 
     def apply_kron(self, dims, F, U):
+        F, U = _rowmajor(F, U)
         B, n, _ = U.shape
         Y = torch.empty_like(U)
         tmp = torch.empty_like(U)
@@ -523,6 +538,7 @@ class SpinEngine:
     def liouvillian(self, H, A=None, rates=None):
         """L (B, n^2, n^2) of Hamiltonians H (B, n, n) and jump operators
         A (B, k, n, n) with rates (B, k)."""
+        H, A, rates = _rowmajor(H, A, rates)
         B, n, _ = H.shape
         k = 0 if A is None else A.shape[1]
         Lm = torch.empty((B, n * n, n * n), dtype=torch.complex128, device=self.device)
@@ -534,6 +550,7 @@ class SpinEngine:
     def lindblad_modes(self, Lm, r, u=None, want_c=False, want_V=False):
         """lambda (B, N), amp (B, k, N), c (B, N) or None, V (B, N, N) or None;
         Lm is destroyed.  Raises LinAlgError if a QR iteration did not converge."""
+        Lm, r, u = _rowmajor(Lm, r, u)
         B, Nl, _ = Lm.shape
         k = 0 if u is None else u.shape[1]
         dev = self.device
