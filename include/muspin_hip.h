@@ -130,6 +130,20 @@ size_t msp_eigh_workspace(int n, int batch, int method);
 int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
              int method, void *d_work, size_t work_bytes, void *stream);
 
+/* msp_eigh for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  The divide-and-conquer workspace is a
+ * library-owned static device array of MSP_NOALLOC_SCRATCH_BYTES, one per
+ * device, held until the process ends; a call whose msp_eigh_workspace(n,
+ * batch, MSP_EIGH_DC) exceeds it fails with MSP_ERR_VALUE (64 MiB: 34
+ * instances at n = 192, one at n = 1024).  Calls no hipMalloc/hipFree, creates
+ * no stream or event, and launches everything on `stream`.  Every call on a
+ * device shares that device's array: calls that overlap on one device race,
+ * so serialise them (one stream, or events).  Same results as msp_eigh. */
+#define MSP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int msp_eigh_noalloc(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
+                     int method, void *stream);
+
 /* Y[b] = (v[b].S_mu (x) 1) U[b] with v = d_vec (batch x 3) and the muon
  * position/dims of d_systems[d_sys_id[b]].  pauli = 1 uses sigma = 2S. */
 int msp_apply_muon_op(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,

@@ -26,6 +26,7 @@ MSP_EIGH_JACOBI = 1
 MSP_EIGH_DC = 2
 
 MSP_MAX_SPINS = 16
+MSP_NOALLOC_SCRATCH_BYTES = 64 << 20     # static scratch of msp_eigh_noalloc, per device
 
 # numpy mirrors of the C structs (natural alignment, no packing)
 SYSTEM_DTYPE = np.dtype([
@@ -54,6 +55,7 @@ SIGNATURES = {
     "msp_build_h_lower": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _P, _P]),
     "msp_eigh_workspace": (_SZ, [_I, _I, _I]),
     "msp_eigh": (_I, [_I, _I, _P, _P, _P, _P, _I, _P, _SZ, _P]),
+    "msp_eigh_noalloc": (_I, [_I, _I, _P, _P, _P, _P, _I, _P]),
     "msp_apply_muon_op": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "msp_apply_kron": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),
     "msp_gemm_cn": (_I, [_I, _I, _P, _P, _P, _P]),

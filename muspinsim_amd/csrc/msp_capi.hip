@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/muspin_hip.h"
 #include "msp_kernels.h"
 
@@ -33,6 +35,48 @@ constexpr int kMaxDim = 2048;
 constexpr int kJacobiMax = 64;
 constexpr int kMaxLiouville = 1024;      // N = n^2 of the Lindbladian path
 constexpr int kSchurIts = 30;            // QR sweeps per eigenvalue
+
+}  // namespace
+
+// Workspace of msp_eigh_noalloc: a library-owned device array, one copy per
+// device (each GPU the library runs on holds it until the process ends).  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// only the per-device address from hipGetSymbolAddress is passed to kernels.
+static_assert(MSP_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) double scratch[MSP_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current; nullptr (and the message set) on failure.
+double *scratch_ptr(const char *where)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        hip_status(e, where);
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        fail(MSP_ERR_UNSUPPORTED, "%s: device %d beyond %d", where, dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            hip_status(e == hipSuccess ? hipErrorInvalidSymbol : e, where);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
 
 }  // namespace
 
@@ -165,6 +209,36 @@ int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d
                     work_bytes);
     return hip_status(launch_eigh_dc(n, batch, d_H, d_E, d_U, d_info, (double *)d_work, S(stream)),
                       "msp_eigh(dc)");
+}
+
+int msp_eigh_noalloc(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
+                     int method, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_eigh_noalloc: n = %d out of range", n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_eigh_noalloc: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_H || !d_E || !d_U || !d_info) return fail(MSP_ERR_VALUE, "msp_eigh_noalloc: null pointer");
+    if (method == MSP_EIGH_AUTO) method = (n <= kJacobiMax) ? MSP_EIGH_JACOBI : MSP_EIGH_DC;
+    if (method == MSP_EIGH_JACOBI) {
+        if (n > kJacobiMax)
+            return fail(MSP_ERR_UNSUPPORTED,
+                        "msp_eigh_noalloc: Jacobi path holds n <= %d in LDS (n = %d)", kJacobiMax, n);
+        return hip_status(launch_eigh_jacobi(n, batch, d_H, d_E, d_U, d_info, 40, S(stream)),
+                          "msp_eigh_noalloc(jacobi)");
+    }
+    if (method != MSP_EIGH_DC)
+        return fail(MSP_ERR_VALUE, "msp_eigh_noalloc: unknown method %d", method);
+    const size_t need = msp_eigh_workspace(n, batch, MSP_EIGH_DC);
+    if (need > (size_t)MSP_NOALLOC_SCRATCH_BYTES)
+        return fail(MSP_ERR_VALUE,
+                    "msp_eigh_noalloc: n = %d x batch = %d needs %zu bytes of workspace, more than "
+                    "the %zu-byte static scratch (split the batch, or use msp_eigh)",
+                    n, batch, need, (size_t)MSP_NOALLOC_SCRATCH_BYTES);
+    double *work = scratch_ptr("msp_eigh_noalloc");
+    if (!work) return MSP_ERR_HIP;
+    // every launch on `stream`: no side stream and no events
+    return hip_status(launch_eigh_dc(n, batch, d_H, d_E, d_U, d_info, work, S(stream), false),
+                      "msp_eigh_noalloc(dc)");
 }
 
 int msp_apply_muon_op(int n, int batch, const MspSystem *d_systems, const int32_t *d_sys_id,
