@@ -130,6 +130,18 @@ size_t msp_eigh_workspace(int n, int batch, int method);
 int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
              int method, void *d_work, size_t work_bytes, void *stream);
 
+/* First stage of the mixed-precision eigendecomposition (replaces the
+ * numpy.linalg.eigh of Hermitian.diag, spinop.py:51-82, on the mixed path;
+ * 2 <= n <= 192): Householder tridiagonalisation in FP32 with the whole lower
+ * triangle of each H resident in LDS, then the divide and conquer and the
+ * back-transformation of msp_eigh on its reflectors.  d_H (lower triangle
+ * read) is preserved.  The eigenpairs carry FP32-level errors (backward error
+ * of order 1e-6 ||H||); msp_refine_residual and msp_refine_correction bring
+ * them to FP64 level. */
+size_t msp_eigh_mixed_workspace(int n, int batch);
+int msp_eigh_mixed(int n, int batch, const double *d_H, double *d_E, double *d_U, int32_t *d_info,
+                   void *d_work, size_t work_bytes, void *stream);
+
 /* msp_eigh for callers that cannot allocate device memory: code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
  * counterpart in the reference.  The divide-and-conquer workspace is a

@@ -211,6 +211,38 @@ int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d
                       "msp_eigh(dc)");
 }
 
+size_t msp_eigh_mixed_workspace(int n, int batch)
+{
+    if (n < 1 || n > 192 || batch < 1) return 0;
+    return (dc_workspace_doubles(n) * sizeof(double) + (size_t)n * n * 2 * sizeof(double)) *
+           (size_t)batch;
+}
+
+int msp_eigh_mixed(int n, int batch, const double *d_H, double *d_E, double *d_U, int32_t *d_info,
+                   void *d_work, size_t work_bytes, void *stream)
+{
+    if (n < 2 || n > 192)
+        return fail(MSP_ERR_UNSUPPORTED, "msp_eigh_mixed: n = %d outside 2..192 (LDS-resident FP32 "
+                    "reduction)", n);
+    if (batch < 0) return fail(MSP_ERR_VALUE, "msp_eigh_mixed: batch < 0");
+    if (batch == 0) return MSP_OK;
+    if (!d_H || !d_E || !d_U || !d_info) return fail(MSP_ERR_VALUE, "msp_eigh_mixed: null pointer");
+    const size_t need = msp_eigh_mixed_workspace(n, batch);
+    if (!d_work || work_bytes < need)
+        return fail(MSP_ERR_VALUE, "msp_eigh_mixed: workspace of %zu bytes needed, %zu given", need,
+                    work_bytes);
+    double *work = (double *)d_work;
+    double *R = work + dc_workspace_doubles(n) * (size_t)batch;
+    hipError_t e;
+    {
+        ProfScope ps_("k_tridiag32", S(stream));
+        e = launch_tridiag32(n, batch, d_H, R, d_E, work, S(stream));
+    }
+    if (e != hipSuccess) return hip_status(e, "msp_eigh_mixed(tridiag32)");
+    return hip_status(launch_eigh_from_tridiag(n, batch, R, d_E, d_U, d_info, work, S(stream)),
+                      "msp_eigh_mixed");
+}
+
 int msp_eigh_noalloc(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d_info,
                      int method, void *stream)
 {

@@ -41,6 +41,16 @@ hipError_t launch_eigh_jacobi(int n, int batch, const double *H, double *E, doub
 // launch on s and creates no events.
 hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int32_t *info,
                           double *work, hipStream_t s, bool overlap = true);
+// The divide and conquer and the back-transformation after a tridiagonal
+// reduction: d in E, e and tau in the workspace, reflector tails below the
+// subdiagonal of R (row-major n x n per instance, read only).
+hipError_t launch_eigh_from_tridiag(int n, int batch, const double *R, double *E, double *U,
+                                    int32_t *info, double *work, hipStream_t s, bool overlap = true);
+// FP32 LDS-resident tridiagonalisation (n <= 192) with the same outputs as
+// the FP64 one: reflector tails to R, d to E, e and tau to the workspace.
+size_t tridiag32_lds_bytes(int n);
+hipError_t launch_tridiag32(int n, int batch, const double *H, double *R, double *E, double *work,
+                            hipStream_t s);
 hipError_t launch_apply_muon(int n, int batch, const MspSystem *systems, const int32_t *sys_id,
                              const double *vec, int pauli, const double *U, double *Y,
                              hipStream_t s);

@@ -1061,6 +1061,14 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     }
     if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
+    return launch_eigh_from_tridiag(n, batch, H, E, U, info, work, s, overlap);
+}
+
+hipError_t launch_eigh_from_tridiag(int n, int batch, const double *R, double *E, double *U,
+                                    int32_t *info, double *work, hipStream_t s, bool overlap)
+{
+    double *H = (double *)R;     // read only from here on
+    hipError_t err = hipSuccess;
     // The WY factors need only the reflectors, so they are formed on the side
     // stream while the divide and conquer runs (it depends on d and e only);
     // the back-transformation joins both.  Events are per call, so
@@ -1069,13 +1077,13 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     hipEvent_t ev_td = nullptr, ev_ft = nullptr;
     if (s2 && (hipEventCreateWithFlags(&ev_td, hipEventDisableTiming) != hipSuccess ||
                hipEventCreateWithFlags(&ev_ft, hipEventDisableTiming) != hipSuccess)) {
-        if (ev_td) hipEventDestroy(ev_td);
+        if (ev_td) (void)hipEventDestroy(ev_td);
         ev_td = nullptr;
         s2 = nullptr;
     }
     err = eigh_dc_middle(n, batch, H, E, info, work, s, s2, ev_td, ev_ft);
-    if (ev_td) hipEventDestroy(ev_td);
-    if (ev_ft) hipEventDestroy(ev_ft);
+    if (ev_td) (void)hipEventDestroy(ev_td);
+    if (ev_ft) (void)hipEventDestroy(ev_ft);
     if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (n <= 1024) {
