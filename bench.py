@@ -23,7 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3      # dense FP32 matrix (and vector) peak of MI355X
 HBM_PEAK_GBS = 8000.0
+# kernels whose algorithmic work is priced against the FP32 peak
+FP32_KERNELS = ("k_timesum_mfma", "k_tridiag32")
 
 
 def algorithmic(kernel, n, ntimes, real_mode=True):
@@ -46,6 +49,14 @@ def algorithmic(kernel, n, ntimes, real_mode=True):
         return "flop", 4.0 / 3.0 * n ** 3
     if kernel == "k_timesum":
         return "flop", (4.0 if real_mode else 8.0) * n * n * ntimes
+    if kernel == "k_timesum_mfma":
+        # Y = C [cos | sin]: n x n x 2T real multiply-adds
+        return "flop", 4.0 * n * n * ntimes
+    if kernel == "k_tridiag32":
+        return "flop", 16.0 / 3.0 * n ** 3
+    if kernel == "k_phase_table":
+        # cos and sin tables, FP32, written once
+        return "byte", 8.0 * n * ntimes
     if kernel == "k_integral":
         return "flop", 20.0 * n * n
     if kernel == "k_build_h":
@@ -88,6 +99,43 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def time_plan(eng, plan, args, profile=True):
+    """W untimed steps, then K timed steps bracketed by synchronisations;
+    per-kernel HIP-event times of the timed steps (graph replays record no
+    events, so with --graph one extra eager step supplies them), and one more
+    untimed step without event bracketing for the overhead-free rate."""
+    import torch
+    from muspinsim_amd import _native as N
+    run = eng.execute_graph if args.graph else eng.execute
+    for _ in range(args.warmup):
+        run(plan)
+    torch.cuda.synchronize()
+    N.profile_report()                      # clear
+    N.load_library().msp_profile_enable(1 if (profile and not args.graph) else 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run(plan)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    results = plan.results.cpu().numpy()
+    rows = plan.rows.cpu().numpy()
+    prof_steps = args.steps
+    if profile and args.graph:
+        N.load_library().msp_profile_enable(1)
+        eng.execute(plan)
+        torch.cuda.synchronize()
+        prof_steps = 1
+    N.load_library().msp_profile_enable(0)
+    prof = N.profile_report()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    run(plan)
+    torch.cuda.synchronize()
+    dt_clean = time.perf_counter() - t1
+    return dt, results, rows, prof, prof_steps, dt_clean
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +149,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="instances per launch (0 = auto)")
     ap.add_argument("--chunk-gib", type=float, default=0.0,
                     help="device memory budget of one chunk in GiB (0 = engine default)")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="mixed: time sums of the real routes on FP32 matrix cores; the plain "
+                         "FP64 path is timed beside it for the speedup")
     ap.add_argument("--graph", action="store_true",
                     help="replay the plan from a captured HIP graph (SpinEngine.execute_graph)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -171,40 +222,21 @@ def main():
         kw["max_chunk"] = args.chunk
     if args.chunk_gib:
         kw["chunk_bytes"] = int(args.chunk_gib * (1 << 30))
-    eng = SpinEngine("cuda:0", **kw)
+    eng = SpinEngine("cuda:0", precision=args.precision, **kw)
     plan = eng.prepare(systems, batch)
-    run = eng.execute_graph if args.graph else eng.execute
-    for _ in range(args.warmup):
-        run(plan)
-    torch.cuda.synchronize()
-    N.profile_report()                      # clear
-    # per-kernel HIP-event times come from the timed steps themselves; graph
-    # replays record no events, so with --graph one extra eager step
-    # (untimed) supplies them
-    N.load_library().msp_profile_enable(0 if args.graph else 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run(plan)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    results = plan.results.cpu().numpy()
-    rows = plan.rows.cpu().numpy()
-    prof_steps = args.steps
-    if args.graph:
-        N.load_library().msp_profile_enable(1)
-        eng.execute(plan)
-        torch.cuda.synchronize()
-        prof_steps = 1
-    N.load_library().msp_profile_enable(0)
-    prof = N.profile_report()
-
-    # untimed pass without event bracketing, to report the overhead-free rate
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    run(plan)
-    torch.cuda.synchronize()
-    dt_clean = time.perf_counter() - t1
+    dt, results, rows, prof, prof_steps, dt_clean = time_plan(eng, plan, args)
+    plain = None
+    if args.precision != "fp64":
+        # the plain FP64 path on the same workload, same steps, for the speedup
+        del plan
+        torch.cuda.empty_cache()
+        eng64 = SpinEngine("cuda:0", precision="fp64", **kw)
+        plan64 = eng64.prepare(systems, batch)
+        dt64, res64, rows64, _, _, _ = time_plan(eng64, plan64, args, profile=False)
+        plain = {"value": ninst * args.steps / dt64, "ms_per_step": dt64 / args.steps * 1e3,
+                 "max_abs_dP_mixed_vs_fp64_gpu_all_instances": float(np.abs(rows - rows64).max())}
+        del plan64
+        torch.cuda.empty_cache()
 
     value = ninst * args.steps / dt
     # dominant kernel and its roofline
@@ -219,8 +251,9 @@ def main():
     avg_s = tot_ms / launches / 1e3
     if unit == "flop":
         achieved = per_inst * inst_per_launch / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
+        peak = FP32_PEAK_TFLOPS if kname in FP32_KERNELS else FP64_PEAK_TFLOPS
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
     else:
         achieved = per_inst * inst_per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -246,11 +279,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32-mfma",
         "data": "synthetic (seeded couplings and orientations)",
         "config": {"workload": wname, "launch": "hip-graph replay" if args.graph else "stream", "hilbert_dim": n if dims.min() == n else
                    f"{int(dims.min())}-{n}", "instances": ninst,
-                   "precision": "fp64-plain",
+                   "precision": ("fp64-plain" if args.precision == "fp64" else
+                                 "mixed: FP64 eigensolver and basis change, time sum on FP32 "
+                                 "MFMA with FP64 phase reduction and FP64 row sums"),
                    "eigensolver": "hessenberg+complex-schur (Liouvillian)"
                    if any(getattr(sy, "dissip", None) for sy in systems)
                    else "householder+divide-and-conquer" if dims.min() > 64
@@ -262,6 +297,8 @@ def main():
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "gpu_vs_cpu_full_host_bound": (value / cpu["full_host_linear_bound"]) if cpu else None,
+        "fp64_plain_gpu": plain,
+        "speedup_vs_fp64_plain_gpu": (value / plain["value"]) if plain else None,
         "results_checksum": float(np.sum(results)),
     }
     if args.workload == "c5":

@@ -212,6 +212,19 @@ int msp_timesum(int n, int batch, int ntimes, const double *d_E, const double *d
                 int real_mode, const double *d_times, const double *d_w, int out_complex,
                 double *d_out, void *stream);
 
+/* Mixed-precision msp_timesum for a real symmetric C (real_mode = 1, the
+ * fast_evolve and isotropic routes: Hamiltonian.fast_evolve's cosine sum,
+ * hamiltonian.py:207-217 / cython/parallel.pyx:59-67, and evolve's time loop,
+ * hamiltonian.py:87-107): d_out[b*T + t] = w_b sum_ij C_ij cos(2 pi (E_i - E_j) t)
+ * with the phase arguments range-reduced in FP64, their sine and cosine in
+ * FP32, the sum over j on FP32 matrix cores and the sum over i in FP64.
+ * Errors are of order 1e-7 sum_ij |C_ij|.  d_work holds the phase table
+ * (msp_timesum_mixed_workspace bytes); batch <= 65535. */
+size_t msp_timesum_mixed_workspace(int n, int batch, int ntimes);
+int msp_timesum_mixed(int n, int batch, int ntimes, const double *d_E, const double *d_C,
+                      const double *d_times, const double *d_w, double *d_out, void *d_work,
+                      size_t work_bytes, void *stream);
+
 /* I_b = sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)) * out_scale.
  * out_complex = 0: d_out[b] = w_b Re I_b; 1: complex, unweighted. */
 int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode,

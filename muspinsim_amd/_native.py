@@ -57,6 +57,8 @@ SIGNATURES = {
     "msp_eigh": (_I, [_I, _I, _P, _P, _P, _P, _I, _P, _SZ, _P]),
     "msp_eigh_noalloc": (_I, [_I, _I, _P, _P, _P, _P, _I, _P]),
     "msp_eigh_mixed_workspace": (_SZ, [_I, _I]),
+    "msp_timesum_mixed_workspace": (_SZ, [_I, _I, _I]),
+    "msp_timesum_mixed": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "msp_eigh_mixed": (_I, [_I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "msp_apply_muon_op": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "msp_apply_kron": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),

@@ -48,6 +48,14 @@ hipError_t launch_eigh_from_tridiag(int n, int batch, const double *R, double *E
                                     int32_t *info, double *work, hipStream_t s, bool overlap = true);
 // FP32 LDS-resident tridiagonalisation (n <= 192) with the same outputs as
 // the FP64 one: reflector tails to R, d to E, e and tau to the workspace.
+// Time sum of a real symmetric C on FP32 matrix cores: phase table (FP64
+// range reduction, FP32 sin/cos) in `work`, then Y = C [cos | sin] and the
+// contraction with the phases per 64-time tile.
+int timesum_mfma_ldt(int ntimes);
+size_t timesum_mfma_workspace(int n, int batch, int ntimes);
+hipError_t launch_timesum_mfma(int n, int batch, int ntimes, const double *E, const double *C,
+                               const double *times, const double *w, double *out, void *work,
+                               hipStream_t s);
 size_t tridiag32_lds_bytes(int n);
 hipError_t launch_tridiag32(int n, int batch, const double *H, double *R, double *E, double *work,
                             hipStream_t s);

@@ -127,13 +127,22 @@ class SpinEngine:
     # A chunk's buffers take up to 32 GiB of the 288 GiB of HBM: at n = 192
     # that is 8192 instances, 16 full residency waves of the two-per-CU
     # tridiagonalisation, so partial last waves and launch gaps are rare.
+    # precision "fp64": every stage in FP64 (the drop-in, max|dP| <= 1e-10
+    # against the FP64 oracle); "mixed": the time sums of the real routes run
+    # on FP32 matrix cores with FP64 phase reduction and FP64 sums over rows
+    # (msp_timesum_mixed, max|dP| of order 1e-7), everything else FP64.
+    PRECISIONS = ("fp64", "mixed")
+
     def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=32 << 30,
-                 max_chunk=8192, base_bytes=1 << 30):
+                 max_chunk=8192, base_bytes=1 << 30, precision="fp64"):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("SpinEngine needs a HIP device (torch 'cuda' device on ROCm)")
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {self.PRECISIONS}")
         self.lib = N.lib()
         self.method = method
+        self.precision = precision
         self.chunk_bytes = chunk_bytes
         self.max_chunk = max_chunk
         self.base_bytes = base_bytes
@@ -324,7 +333,7 @@ class SpinEngine:
                 else:
                     cs = self.chunk_size(n, route == "general")
                 g = _Group()
-                g.n, g.route = n, route
+                g.n, g.route, g.width = n, route, width
                 g.cs = min(cs, idx.size)
                 g.idx = idx
                 g.d_idx = self._dev(idx.astype(np.int64), torch.int64)
@@ -393,6 +402,12 @@ class SpinEngine:
         g.work = torch.empty((max(g.wsb, 8) // 8,), dtype=torch.float64, device=dev)
         real = g.route != "general"
         g.C = torch.empty((cs, n, n), dtype=torch.float64 if real else torch.complex128, device=dev)
+        g.tswork, g.tswsb = None, 0
+        # (below n = 48 the time sum is a few microseconds either way and the
+        # phase-table launch would cost more than it saves)
+        if self.precision == "mixed" and real and g.width > 1 and n >= 48:
+            g.tswsb = self.lib.msp_timesum_mixed_workspace(n, cs, g.width)
+            g.tswork = torch.empty((max(g.tswsb, 8) // 8,), dtype=torch.float64, device=dev)
         if not real:
             g.R = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
             g.T2 = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
@@ -499,6 +514,10 @@ class SpinEngine:
                 if batch.y == "integral":
                     N.check(L.msp_integral(n, nb, g.E.data_ptr(), g.C.data_ptr(), real, MU_TAU,
                                            1.0 / MU_TAU, p(g.w), 0, out.data_ptr(), st))
+                elif g.tswork is not None:
+                    N.check(L.msp_timesum_mixed(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(),
+                                                plan.times.data_ptr(), p(g.w), out.data_ptr(),
+                                                g.tswork.data_ptr(), g.tswsb, st))
                 else:
                     N.check(L.msp_timesum(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(), real,
                                           plan.times.data_ptr(), p(g.w), 0, out.data_ptr(), st))

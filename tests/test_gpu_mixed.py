@@ -76,3 +76,54 @@ def test_eigh_mixed_alc_hamiltonians(gpu_lib):
         nrm = np.abs(Er).max()
         assert np.abs(E[b] - Er).max() <= 5e-5 * nrm
         assert np.abs(H[b] @ U[b] - U[b] * E[b]).max() <= 5e-5 * nrm
+
+
+@pytest.mark.parametrize("n,T,batch", [(4, 7, 2), (16, 130, 3), (64, 200, 11), (70, 65, 2),
+                                       (192, 300, 3), (512, 100, 2)])
+def test_timesum_mixed_vs_fp64(gpu_lib, n, T, batch):
+    """msp_timesum_mixed against the FP64 time sum of the same real symmetric
+    C (numpy, and msp_timesum): |dP| <= 1e-6 sum_ij |C_ij| per time, with
+    phases over 0..20 us at eigenvalue spreads of +-2 GHz."""
+    from muspinsim_amd import _native as N
+    rng = np.random.default_rng(n + T)
+    E = np.sort(rng.normal(size=(batch, n)) * 2000.0, axis=1)
+    G = rng.normal(size=(batch, n, n))
+    C = np.ascontiguousarray((G + np.swapaxes(G, 1, 2)) / (2 * n * n))
+    t = np.linspace(0, 20, T)
+    w = rng.uniform(0.5, 2.0, size=batch)
+    dE, dC, dt, dw = (_dev(x, torch.float64) for x in (E, C, t, w))
+    out = torch.empty((batch, T), dtype=torch.float64, device="cuda")
+    ws = gpu_lib.msp_timesum_mixed_workspace(n, batch, T)
+    work = torch.empty((ws // 8,), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    N.check(gpu_lib.msp_timesum_mixed(n, batch, T, dE.data_ptr(), dC.data_ptr(), dt.data_ptr(),
+                                      dw.data_ptr(), out.data_ptr(), work.data_ptr(), ws, st))
+    ref64 = torch.empty_like(out)
+    N.check(gpu_lib.msp_timesum(n, batch, T, dE.data_ptr(), dC.data_ptr(), 1, dt.data_ptr(),
+                                dw.data_ptr(), 0, ref64.data_ptr(), st))
+    got, g64 = out.cpu().numpy(), ref64.cpu().numpy()
+    for b in range(batch):
+        ph = np.cos(2 * np.pi * (E[b][:, None] - E[b][None, :])[None] * t[:, None, None])
+        ref = w[b] * np.einsum("tij,ij->t", ph, C[b])
+        scale = w[b] * np.abs(C[b]).sum()
+        # FP64 phases of up to 2 pi (E_max - E_min) t_max ~ 1e6 rad carry
+        # relative errors of eps * |phase| (numpy's cos included)
+        phmax = 2 * np.pi * (E[b].max() - E[b].min()) * t.max()
+        assert np.abs(g64[b] - ref).max() <= 4e-16 * max(phmax, 1.0) * scale
+        assert np.abs(got[b] - ref).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("cfg", ["c3", "c5", "c1"])
+def test_mixed_engine_end_to_end(gpu_lib, cfg):
+    """SpinEngine(precision='mixed') against the FP64 oracle at max|dP| <= 1e-6
+    on reduced forms of the time-series configurations (fast route)."""
+    from muspinsim_amd import workloads as W
+    from muspinsim_amd.engine import SpinEngine
+    from oracle import pipeline as opipe
+    systems, batch = {"c3": lambda: W.c3_tf_stress(nori=3, ntimes=400),
+                      "c5": lambda: W.c5_fit(nori=16, ntimes=500),
+                      "c1": lambda: W.c1_zero_field_powder(nori=20, ntimes=300)}[cfg]()
+    res = SpinEngine("cuda:0", precision="mixed").run(systems, batch)
+    ref = opipe.run(systems, batch)
+    err = np.abs(res - ref).max()
+    assert err <= 1e-6, f"max |dP| = {err:.3e}"
