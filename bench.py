@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="mixed: time sums of the real routes on FP32 matrix cores; the plain "
                          "FP64 path is timed beside it for the speedup")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="with --precision mixed, skip the FP64 comparison run (profiling)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the plan from a captured HIP graph (SpinEngine.execute_graph)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -226,7 +228,7 @@ def main():
     plan = eng.prepare(systems, batch)
     dt, results, rows, prof, prof_steps, dt_clean = time_plan(eng, plan, args)
     plain = None
-    if args.precision != "fp64":
+    if args.precision != "fp64" and not args.no_plain:
         # the plain FP64 path on the same workload, same steps, for the speedup
         del plan
         torch.cuda.empty_cache()
