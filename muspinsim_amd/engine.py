@@ -134,7 +134,7 @@ class SpinEngine:
     PRECISIONS = ("fp64", "mixed")
 
     def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=32 << 30,
-                 max_chunk=8192, base_bytes=1 << 30, precision="fp64"):
+                 max_chunk=8192, base_bytes=1 << 30, precision="fp64", group_streams=4):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("SpinEngine needs a HIP device (torch 'cuda' device on ROCm)")
@@ -143,6 +143,7 @@ class SpinEngine:
         self.lib = N.lib()
         self.method = method
         self.precision = precision
+        self.group_streams = int(group_streams)
         self.chunk_bytes = chunk_bytes
         self.max_chunk = max_chunk
         self.base_bytes = base_bytes
@@ -334,6 +335,7 @@ class SpinEngine:
                     cs = self.chunk_size(n, route == "general")
                 g = _Group()
                 g.n, g.route, g.width = n, route, width
+                g.cost = float(idx.size) * float(n) ** 3
                 g.cs = min(cs, idx.size)
                 g.idx = idx
                 g.d_idx = self._dev(idx.astype(np.int64), torch.int64)
@@ -419,113 +421,151 @@ class SpinEngine:
         priority; every call replays it.  Returns the results tensor, which
         each replay overwrites; plan.checks refresh with it."""
         if getattr(plan, "graph", None) is None:
-            self.execute(plan)
+            # captured on one stream: capturing the multi-stream form of a
+            # several-group plan ended in a host crash inside capture_end, so
+            # graphs keep the single-stream launch order
+            self.execute(plan, concurrent=False)
             torch.cuda.synchronize(self.device)
             g = torch.cuda.CUDAGraph()
             cap = torch.cuda.Stream(self.device)
             cap.wait_stream(torch.cuda.current_stream(self.device))
             with torch.cuda.graph(g, stream=cap, capture_error_mode="thread_local"):
-                res = self.execute(plan)
+                res = self.execute(plan, concurrent=False)
             torch.cuda.current_stream(self.device).wait_stream(cap)
             plan.graph, plan.graph_results = g, res
         plan.graph.replay()
         return plan.graph_results
 
-    def execute(self, plan):
+    def _stream_pool(self):
+        """Streams of default priority on which dimension groups run side by
+        side (created once per engine)."""
+        if getattr(self, "_pool", None) is None:
+            self._pool = [torch.cuda.Stream(self.device) for _ in range(self.group_streams)]
+        return self._pool
+
+    def execute(self, plan, concurrent=True):
         """Launch every stage for every chunk; returns the device results
-        tensor (n_slots, width).  Asynchronous on the current stream."""
+        tensor (n_slots, width).  Asynchronous on the current stream.
+
+        Dimension groups are independent until the final accumulation, so a
+        plan with several groups (uneven batches) runs them on a pool of
+        streams forked from and joined back to the current stream, the
+        costliest group first: one group's one-workgroup-per-instance
+        launches (a few hundred workgroups at the largest n) leave CUs free
+        for the others."""
         L = self.lib
-        st = self.stream()
+        main = torch.cuda.current_stream(self.device)
+        st = main.cuda_stream
         batch = plan.batch
         plan.results.zero_()
         plan.checks = []
-        for g in plan.groups:
-            n = g.n
-            if g.base is not None:
-                N.check(L.msp_build_h(n, g.base.shape[0], g.base_sys.data_ptr(),
-                                      plan.d_terms.data_ptr(), g.base_ids.data_ptr(), None, None, 0,
-                                      g.base.data_ptr(), st))
-            for c0 in range(0, g.idx.size, g.cs):
-                nb = min(g.cs, g.idx.size - c0)
-                sl = slice(c0, c0 + nb)
-
-                def p(t):
-                    return t[sl].data_ptr()
-
-                N.check(L.msp_rotate_fields(nb, p(g.q), p(g.B), p(g.Bint), p(g.p),
-                                            g.Br.data_ptr(), g.pr.data_ptr(), st))
-                # the D&C eigensolver (n <= 1024) reads the lower triangle only
-                build = (L.msp_build_h_lower if g.route != "lindblad" and self._dc_lower(n)
-                         else L.msp_build_h)
-                if g.base is not None:
-                    N.check(build(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
-                                  p(g.sid_local), g.Br.data_ptr(), g.base.data_ptr(), 1,
-                                  g.H.data_ptr(), st))
-                else:
-                    N.check(build(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
-                                  p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
-                if g.route == "lindblad":
-                    self._execute_lindblad(plan, g, sl, nb, st)
-                    continue
-                N.check(L.msp_eigh(n, nb, g.H.data_ptr(), g.E.data_ptr(), g.U.data_ptr(),
-                                   g.info.data_ptr(), self.method, g.work.data_ptr(), g.wsb, st))
-                plan.checks.append(g.info[:nb].amax())
-                Y = g.H   # destroyed by the eigensolver; reused as scratch
-                pauli = 1 if g.route == "fast" else 0
-                out = torch.empty((nb, plan.width), dtype=torch.float64, device=self.device)
-                if batch.y == "integral" and g.route in ("fast", "iso"):
-                    # O is Hermitian and the integral weights |O_ij|^2 by an
-                    # even kernel: the upper block triangle of O suffices
-                    if g.mu_half:
-                        N.check(L.msp_gemm_muon_upper(n, nb, plan.d_sys.data_ptr(), p(g.sid),
-                                                      g.pr.data_ptr(), pauli, g.U.data_ptr(),
-                                                      g.O.data_ptr(), st))
-                    else:
-                        N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
-                                                    g.pr.data_ptr(), pauli, g.U.data_ptr(),
-                                                    Y.data_ptr(), st))
-                        N.check(L.msp_gemm_cn_upper(n, nb, g.U.data_ptr(), Y.data_ptr(),
-                                                    g.O.data_ptr(), st))
-                    alpha, beta = ((1.0 / (4.0 * g.d_other), 0.0) if g.route == "fast"
-                                   else (2.0 / n, 1.0 / n))
-                    N.check(L.msp_integral_obs(n, nb, g.E.data_ptr(), g.O.data_ptr(), alpha, beta,
-                                               MU_TAU, 1.0 / MU_TAU, p(g.w), out.data_ptr(), st))
-                    plan.rows.index_copy_(0, g.d_idx[sl], out)
-                    continue
-                N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
-                                            g.pr.data_ptr(), pauli, g.U.data_ptr(), Y.data_ptr(), st))
-                N.check(L.msp_gemm_cn(n, nb, g.U.data_ptr(), Y.data_ptr(), g.O.data_ptr(), st))
-                if g.route == "fast":
-                    N.check(L.msp_form_c(n, nb, None, g.O.data_ptr(), 1, 1.0 / (4.0 * g.d_other),
-                                         0.0, g.C.data_ptr(), st))
-                    real = 1
-                elif g.route == "iso":
-                    N.check(L.msp_form_c(n, nb, None, g.O.data_ptr(), 1, 2.0 / n, 1.0 / n,
-                                         g.C.data_ptr(), st))
-                    real = 1
-                else:
-                    hd = ctypes_int_array(g.dims)
-                    N.check(L.msp_apply_kron(n, nb, len(g.dims), hd, p(g.F), g.U.data_ptr(),
-                                             Y.data_ptr(), g.T2.data_ptr(), st))
-                    N.check(L.msp_gemm_cn(n, nb, g.U.data_ptr(), Y.data_ptr(), g.R.data_ptr(), st))
-                    N.check(L.msp_form_c(n, nb, g.R.data_ptr(), g.O.data_ptr(), 0, 0.0, 0.0,
-                                         g.C.data_ptr(), st))
-                    real = 0
-                if batch.y == "integral":
-                    N.check(L.msp_integral(n, nb, g.E.data_ptr(), g.C.data_ptr(), real, MU_TAU,
-                                           1.0 / MU_TAU, p(g.w), 0, out.data_ptr(), st))
-                elif g.tswork is not None:
-                    N.check(L.msp_timesum_mixed(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(),
-                                                plan.times.data_ptr(), p(g.w), out.data_ptr(),
-                                                g.tswork.data_ptr(), g.tswsb, st))
-                else:
-                    N.check(L.msp_timesum(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(), real,
-                                          plan.times.data_ptr(), p(g.w), 0, out.data_ptr(), st))
-                plan.rows.index_copy_(0, g.d_idx[sl], out)
+        groups = plan.groups
+        if concurrent and self.group_streams > 1 and len(groups) > 1:
+            pool = self._stream_pool()
+            order = sorted(range(len(groups)), key=lambda i: -groups[i].cost)
+            used = pool[: min(len(pool), len(groups))]
+            for s in used:
+                s.wait_stream(main)
+            for k, gi in enumerate(order):
+                with torch.cuda.stream(used[k % len(used)]):
+                    self._execute_group(plan, groups[gi])
+            for s in used:
+                main.wait_stream(s)
+        else:
+            for g in groups:
+                self._execute_group(plan, g)
         N.check(L.msp_accumulate(batch.n_slots, plan.width, plan.d_ptr.data_ptr(),
                                  plan.d_order.data_ptr(), plan.rows.data_ptr(), 1.0 / batch.avg_N,
                                  plan.results.data_ptr(), st))
         return plan.results
+
+    def _execute_group(self, plan, g):
+        """Every chunk of one dimension group, on the current stream."""
+        L = self.lib
+        st = self.stream()
+        batch = plan.batch
+        n = g.n
+        if g.base is not None:
+            N.check(L.msp_build_h(n, g.base.shape[0], g.base_sys.data_ptr(),
+                                  plan.d_terms.data_ptr(), g.base_ids.data_ptr(), None, None, 0,
+                                  g.base.data_ptr(), st))
+        for c0 in range(0, g.idx.size, g.cs):
+            nb = min(g.cs, g.idx.size - c0)
+            sl = slice(c0, c0 + nb)
+
+            def p(t):
+                return t[sl].data_ptr()
+
+            N.check(L.msp_rotate_fields(nb, p(g.q), p(g.B), p(g.Bint), p(g.p),
+                                        g.Br.data_ptr(), g.pr.data_ptr(), st))
+            # the D&C eigensolver (n <= 1024) reads the lower triangle only
+            build = (L.msp_build_h_lower if g.route != "lindblad" and self._dc_lower(n)
+                     else L.msp_build_h)
+            if g.base is not None:
+                N.check(build(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
+                              p(g.sid_local), g.Br.data_ptr(), g.base.data_ptr(), 1,
+                              g.H.data_ptr(), st))
+            else:
+                N.check(build(n, nb, plan.d_sys.data_ptr(), plan.d_terms.data_ptr(),
+                              p(g.sid), g.Br.data_ptr(), None, 1, g.H.data_ptr(), st))
+            if g.route == "lindblad":
+                self._execute_lindblad(plan, g, sl, nb, st)
+                continue
+            N.check(L.msp_eigh(n, nb, g.H.data_ptr(), g.E.data_ptr(), g.U.data_ptr(),
+                               g.info.data_ptr(), self.method, g.work.data_ptr(), g.wsb, st))
+            plan.checks.append(g.info[:nb].amax())
+            Y = g.H   # destroyed by the eigensolver; reused as scratch
+            pauli = 1 if g.route == "fast" else 0
+            out = torch.empty((nb, plan.width), dtype=torch.float64, device=self.device)
+            if batch.y == "integral" and g.route in ("fast", "iso"):
+                # O is Hermitian and the integral weights |O_ij|^2 by an
+                # even kernel: the upper block triangle of O suffices
+                if g.mu_half:
+                    N.check(L.msp_gemm_muon_upper(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                                  g.pr.data_ptr(), pauli, g.U.data_ptr(),
+                                                  g.O.data_ptr(), st))
+                else:
+                    N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                                g.pr.data_ptr(), pauli, g.U.data_ptr(),
+                                                Y.data_ptr(), st))
+                    N.check(L.msp_gemm_cn_upper(n, nb, g.U.data_ptr(), Y.data_ptr(),
+                                                g.O.data_ptr(), st))
+                alpha, beta = ((1.0 / (4.0 * g.d_other), 0.0) if g.route == "fast"
+                               else (2.0 / n, 1.0 / n))
+                N.check(L.msp_integral_obs(n, nb, g.E.data_ptr(), g.O.data_ptr(), alpha, beta,
+                                           MU_TAU, 1.0 / MU_TAU, p(g.w), out.data_ptr(), st))
+                plan.rows.index_copy_(0, g.d_idx[sl], out)
+                continue
+            N.check(L.msp_apply_muon_op(n, nb, plan.d_sys.data_ptr(), p(g.sid),
+                                        g.pr.data_ptr(), pauli, g.U.data_ptr(), Y.data_ptr(), st))
+            N.check(L.msp_gemm_cn(n, nb, g.U.data_ptr(), Y.data_ptr(), g.O.data_ptr(), st))
+            if g.route == "fast":
+                N.check(L.msp_form_c(n, nb, None, g.O.data_ptr(), 1, 1.0 / (4.0 * g.d_other),
+                                     0.0, g.C.data_ptr(), st))
+                real = 1
+            elif g.route == "iso":
+                N.check(L.msp_form_c(n, nb, None, g.O.data_ptr(), 1, 2.0 / n, 1.0 / n,
+                                     g.C.data_ptr(), st))
+                real = 1
+            else:
+                hd = ctypes_int_array(g.dims)
+                N.check(L.msp_apply_kron(n, nb, len(g.dims), hd, p(g.F), g.U.data_ptr(),
+                                         Y.data_ptr(), g.T2.data_ptr(), st))
+                N.check(L.msp_gemm_cn(n, nb, g.U.data_ptr(), Y.data_ptr(), g.R.data_ptr(), st))
+                N.check(L.msp_form_c(n, nb, g.R.data_ptr(), g.O.data_ptr(), 0, 0.0, 0.0,
+                                     g.C.data_ptr(), st))
+                real = 0
+            if batch.y == "integral":
+                N.check(L.msp_integral(n, nb, g.E.data_ptr(), g.C.data_ptr(), real, MU_TAU,
+                                       1.0 / MU_TAU, p(g.w), 0, out.data_ptr(), st))
+            elif g.tswork is not None:
+                N.check(L.msp_timesum_mixed(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(),
+                                            plan.times.data_ptr(), p(g.w), out.data_ptr(),
+                                            g.tswork.data_ptr(), g.tswsb, st))
+            else:
+                N.check(L.msp_timesum(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(), real,
+                                      plan.times.data_ptr(), p(g.w), 0, out.data_ptr(), st))
+            plan.rows.index_copy_(0, g.d_idx[sl], out)
 
 
     def _execute_lindblad(self, plan, g, sl, nb, st):
