@@ -283,7 +283,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64+f32-mfma",
         "data": "synthetic (seeded couplings and orientations)",
-        "config": {"workload": wname, "launch": "hip-graph replay" if args.graph else "stream", "hilbert_dim": n if dims.min() == n else
+        "config": {"workload": wname, "launch": "hip-graph replay" if args.graph else "stream",
+                   "tuning": "muspinsim_amd/tuning.json" if eng.tuning else "built-in", "hilbert_dim": n if dims.min() == n else
                    f"{int(dims.min())}-{n}", "instances": ninst,
                    "precision": ("fp64-plain" if args.precision == "fp64" else
                                  "mixed: FP64 eigensolver and basis change, time sum on FP32 "

@@ -134,7 +134,8 @@ class SpinEngine:
     PRECISIONS = ("fp64", "mixed")
 
     def __init__(self, device="cuda:0", method=N.MSP_EIGH_AUTO, chunk_bytes=32 << 30,
-                 max_chunk=8192, base_bytes=1 << 30, precision="fp64", group_streams=4):
+                 max_chunk=None, base_bytes=1 << 30, precision="fp64", group_streams=4,
+                 tuning="default"):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("SpinEngine needs a HIP device (torch 'cuda' device on ROCm)")
@@ -142,6 +143,23 @@ class SpinEngine:
             raise ValueError(f"precision must be one of {self.PRECISIONS}")
         self.lib = N.lib()
         self.method = method
+        # launch choices measured by muspinsim_amd.autotune on this kind of
+        # device: "default" reads muspinsim_amd/tuning.json when it exists,
+        # None uses the built-in choices, a dict or a path is used as given
+        from . import autotune
+        if isinstance(tuning, dict):
+            self.tuning = tuning
+        elif tuning == "default":
+            self.tuning = autotune.load()
+        elif tuning:
+            self.tuning = autotune.load(tuning)
+        else:
+            self.tuning = None
+        t = self.tuning or {}
+        if max_chunk is None:
+            max_chunk = int(t.get("max_chunk", 8192))
+        self.mixed_min_n = int(t.get("mixed_timesum_min_n", 48))
+        self.mixed_min_work = float(t.get("mixed_timesum_min_work", 1e9))
         self.precision = precision
         self.group_streams = int(group_streams)
         self.chunk_bytes = chunk_bytes
@@ -159,20 +177,32 @@ class SpinEngine:
         raw = np.frombuffer(np.ascontiguousarray(structarr).tobytes(), dtype=np.uint8)
         return torch.from_numpy(raw.copy()).to(self.device)
 
-    def per_instance_bytes(self, n, complex_rho):
+    def method_for(self, n, count):
+        """Eigensolver for `count` instances of dimension n: the engine's
+        method when one was given, else the tuned choice (AUTO without one)."""
+        if self.method != N.MSP_EIGH_AUTO:
+            return self.method
+        from . import autotune
+        m = autotune.eigh_method(self.tuning, n, count)
+        if m == N.MSP_EIGH_JACOBI and n > 64:
+            m = N.MSP_EIGH_AUTO
+        return m
+
+    def per_instance_bytes(self, n, complex_rho, method=None):
         mats = 5 if complex_rho else 4
-        ws = N.load_library().msp_eigh_workspace(n, 1, self.method)
+        m = self.method if method is None else method
+        ws = N.load_library().msp_eigh_workspace(n, 1, m)
         return mats * n * n * 16 + ws + n * 8 * 4
 
-    def chunk_size(self, n, complex_rho):
-        per = self.per_instance_bytes(n, complex_rho)
+    def chunk_size(self, n, complex_rho, method=None):
+        per = self.per_instance_bytes(n, complex_rho, method)
         return int(max(1, min(self.max_chunk, self.chunk_bytes // per)))
 
     # ---------------------------------------------------------------- stages
-    def _dc_lower(self, n):
-        """Whether msp_eigh with this engine's method reads only the lower
-        triangle of an n x n input (the D&C path up to n = 1024)."""
-        m = self.method
+    def _dc_lower(self, n, method=None):
+        """Whether msp_eigh with this method reads only the lower triangle of
+        an n x n input (the D&C path up to n = 1024)."""
+        m = self.method if method is None else method
         if m == N.MSP_EIGH_AUTO:
             m = N.MSP_EIGH_JACOBI if n <= 64 else N.MSP_EIGH_DC
         return m == N.MSP_EIGH_DC and n <= 1024
@@ -331,9 +361,12 @@ class SpinEngine:
                             "dissipative systems are limited to n^2 <= 1024 (Liouville space)")
                     cs = int(max(1, min(self.max_chunk, self.chunk_bytes //
                                         (3 * n ** 4 * 16 + 8 * n * n * 16))))
+                    method = N.MSP_EIGH_AUTO
                 else:
-                    cs = self.chunk_size(n, route == "general")
+                    method = self.method_for(n, idx.size)
+                    cs = self.chunk_size(n, route == "general", method)
                 g = _Group()
+                g.method = method
                 g.n, g.route, g.width = n, route, width
                 g.cost = float(idx.size) * float(n) ** 3
                 g.cs = min(cs, idx.size)
@@ -400,14 +433,17 @@ class SpinEngine:
         g.O = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
         g.E = torch.empty((cs, n), dtype=torch.float64, device=dev)
         g.info = torch.empty((cs,), dtype=torch.int32, device=dev)
-        g.wsb = self.lib.msp_eigh_workspace(n, cs, self.method)
+        g.wsb = self.lib.msp_eigh_workspace(n, cs, g.method)
         g.work = torch.empty((max(g.wsb, 8) // 8,), dtype=torch.float64, device=dev)
         real = g.route != "general"
         g.C = torch.empty((cs, n, n), dtype=torch.float64 if real else torch.complex128, device=dev)
         g.tswork, g.tswsb = None, 0
-        # (below n = 48 the time sum is a few microseconds either way and the
-        # phase-table launch would cost more than it saves)
-        if self.precision == "mixed" and real and g.width > 1 and n >= 48:
+        # (below mixed_min_n, tuned, 48 without tuning, the time sum takes a
+        # few microseconds either way and the phase-table launch costs more)
+        # and below about 1e9 multiply-adds per launch the phase-table launch
+        # is not repaid either (few instances, short time axes)
+        if (self.precision == "mixed" and real and g.width > 1 and n >= self.mixed_min_n
+                and float(n) * n * g.width * cs >= self.mixed_min_work):
             g.tswsb = self.lib.msp_timesum_mixed_workspace(n, cs, g.width)
             g.tswork = torch.empty((max(g.tswsb, 8) // 8,), dtype=torch.float64, device=dev)
         if not real:
@@ -499,7 +535,7 @@ class SpinEngine:
             N.check(L.msp_rotate_fields(nb, p(g.q), p(g.B), p(g.Bint), p(g.p),
                                         g.Br.data_ptr(), g.pr.data_ptr(), st))
             # the D&C eigensolver (n <= 1024) reads the lower triangle only
-            build = (L.msp_build_h_lower if g.route != "lindblad" and self._dc_lower(n)
+            build = (L.msp_build_h_lower if g.route != "lindblad" and self._dc_lower(n, g.method)
                      else L.msp_build_h)
             if g.base is not None:
                 N.check(build(n, nb, g.base_sys.data_ptr(), plan.d_terms.data_ptr(),
@@ -512,7 +548,7 @@ class SpinEngine:
                 self._execute_lindblad(plan, g, sl, nb, st)
                 continue
             N.check(L.msp_eigh(n, nb, g.H.data_ptr(), g.E.data_ptr(), g.U.data_ptr(),
-                               g.info.data_ptr(), self.method, g.work.data_ptr(), g.wsb, st))
+                               g.info.data_ptr(), g.method, g.work.data_ptr(), g.wsb, st))
             plan.checks.append(g.info[:nb].amax())
             Y = g.H   # destroyed by the eigensolver; reused as scratch
             pauli = 1 if g.route == "fast" else 0

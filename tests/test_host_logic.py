@@ -60,3 +60,19 @@ def test_c4_dissipative_builder():
     assert all(s.n == 8 and list(s.dissip) == [1] for s in systems)
     assert all(1.0 <= s.dissip[1] <= 100.0 for s in systems)
     assert batch.times.shape == (50,) and batch.avg_N == 16
+
+
+def test_autotune_ladder_and_engine_choice():
+    """The tuned eigensolver ladder (batch thresholds per dimension) and the
+    engine's per-group choice: explicit methods win, Jacobi is never chosen
+    above n = 64, missing entries fall back to AUTO."""
+    from muspinsim_amd import _native as N
+    from muspinsim_amd import autotune
+    t = {"eigh": {"64": [[1, "jacobi"], [1024, "dc"]], "16": [[1, "jacobi"]]}}
+    assert autotune.eigh_method(t, 64, 256) == N.MSP_EIGH_JACOBI
+    assert autotune.eigh_method(t, 64, 1024) == N.MSP_EIGH_DC
+    assert autotune.eigh_method(t, 64, 50000) == N.MSP_EIGH_DC
+    assert autotune.eigh_method(t, 16, 10 ** 6) == N.MSP_EIGH_JACOBI
+    assert autotune.eigh_method(t, 32, 10) == N.MSP_EIGH_AUTO
+    assert autotune.eigh_method(None, 64, 10) == N.MSP_EIGH_AUTO
+    assert autotune.load("/nonexistent/tuning.json") is None
