@@ -61,66 +61,86 @@ __global__ void __launch_bounds__(256) k_phase_table(int n, int ntimes, int ldt,
 // Y = C [c | s] on MFMA and P(t) = sum_i c_i Y^c_i + s_i Y^s_i for one
 // 64-time tile of one instance, looping over 64-row blocks of C.  Wave w
 // owns rows 16 w .. 16 w + 15 of each block and all 64 times (four 16-time
-// column tiles for cos and four for sin: eight 16 x 16 accumulators).
+// column tiles for cos and four for sin: eight 16 x 16 accumulators).  The
+// operands go through LDS in chunks of kKC columns of C (FP64 -> FP32 on
+// the way) and kKC rows of the phase tables, shared by the four waves; the
+// next chunk's loads are issued into registers before the current chunk's
+// matrix-core work, so their latency overlaps it.
+constexpr int kKC = 32;          // K (j) per staged chunk
+constexpr int kLA = kKC + 1;     // padded row of the staged C block
+constexpr int kLB = kTT + 4;     // padded row of a staged phase block
+
 __global__ void __launch_bounds__(256) k_timesum_mfma(int n, int ntimes, int ldt, const double *C,
                                                       const float *Fc, const float *Fs,
                                                       const double *w, double *out)
 {
+    __shared__ float As[64 * kLA];
+    __shared__ float Bc[kKC * kLB], Bs[kKC * kLB];
     __shared__ double red[4][kTT];
     int tt, b;
     msp_xcd_tile(gridDim.x, gridDim.y, tt, b);   // an instance's time tiles share one L2
     const int t0 = tt * kTT;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lq = lane >> 4;
     const double *Cb = C + (size_t)b * n * n;
     const float *Fcb = Fc + (size_t)b * n * ldt;
     const float *Fsb = Fs + (size_t)b * n * ldt;
+    // staging assignment: C block 64 x kKC (8 per thread: row tid / 4,
+    // columns 8 (tid % 4) ..), phase blocks kKC x 64 each (8 per thread:
+    // row tid / 8, times 8 (tid % 8) ..)
+    const int ar = tid >> 2, ac = (tid & 3) * 8;
+    const int br = tid >> 3, bc = (tid & 7) * 8;
     double part[4] = {0.0, 0.0, 0.0, 0.0};        // per column tile, lane's time t0 + 16 ct + li
+    const int nk = (n + kKC - 1) / kKC;
     for (int i0 = 0; i0 < n; i0 += 64) {
-        const int ia = i0 + 16 * wave + li;       // A-operand row of this lane
-        const bool iok = ia < n;
-        const double *Crow = Cb + (size_t)(iok ? ia : 0) * n;
         f4 yc[4], ys[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
             yc[ct] = f4zero();
             ys[ct] = f4zero();
         }
-        int j0 = 0;
-        for (; j0 + 16 <= n; j0 += 16) {
-            float a[4], bc[4][4], bs[4][4];
+        const int ia = i0 + ar;
+        const double *Crow = Cb + (size_t)((ia < n) ? ia : n - 1) * n;
+        double ra[8];
+        float rbc[8], rbs[8];
+        auto load = [&](int j0) {
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const int j = j0 + 4 * h + lq;
-                a[h] = iok ? (float)Crow[j] : 0.f;
-                const float *pc = Fcb + (size_t)j * ldt + t0 + li;
-                const float *ps = Fsb + (size_t)j * ldt + t0 + li;
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct) {
-                    bc[h][ct] = pc[16 * ct];
-                    bs[h][ct] = ps[16 * ct];
-                }
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + ac + u;
+                ra[u] = (ia < n && j < n) ? Crow[j] : 0.0;
             }
+            const int jb = j0 + br;
+            const bool jok = jb < n;
+            const float *pc = Fcb + (size_t)(jok ? jb : 0) * ldt + t0 + bc;
+            const float *ps = Fsb + (size_t)(jok ? jb : 0) * ldt + t0 + bc;
 #pragma unroll
-            for (int h = 0; h < 4; ++h)
+            for (int u = 0; u < 8; ++u) {
+                rbc[u] = jok ? pc[u] : 0.f;
+                rbs[u] = jok ? ps[u] : 0.f;
+            }
+        };
+        load(0);
+        for (int kc = 0; kc < nk; ++kc) {
+            __syncthreads();                      // the previous chunk's reads are done
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                As[ar * kLA + ac + u] = (float)ra[u];
+                Bc[br * kLB + bc + u] = rbc[u];
+                Bs[br * kLB + bc + u] = rbs[u];
+            }
+            __syncthreads();
+            if (kc + 1 < nk) load((kc + 1) * kKC);   // in flight during the MFMAs
+#pragma unroll
+            for (int h = 0; h < kKC / 4; ++h) {
+                const float a = As[(16 * wave + li) * kLA + 4 * h + lq];
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct) {
-                    yc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[h], bc[h][ct], yc[ct], 0, 0, 0);
-                    ys[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[h], bs[h][ct], ys[ct], 0, 0, 0);
+                    const float vc = Bc[(4 * h + lq) * kLB + 16 * ct + li];
+                    const float vs = Bs[(4 * h + lq) * kLB + 16 * ct + li];
+                    yc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, vc, yc[ct], 0, 0, 0);
+                    ys[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, vs, ys[ct], 0, 0, 0);
                 }
-        }
-        for (; j0 < n; j0 += 4) {
-            const int j = j0 + lq;
-            const bool jok = j < n;
-            const float a = (iok && jok) ? (float)Crow[j] : 0.f;
-            const int jc = jok ? j : n - 1;
-            const float *pc = Fcb + (size_t)jc * ldt + t0 + li;
-            const float *ps = Fsb + (size_t)jc * ldt + t0 + li;
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const float vc = jok ? pc[16 * ct] : 0.f, vs = jok ? ps[16 * ct] : 0.f;
-                yc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, vc, yc[ct], 0, 0, 0);
-                ys[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, vs, ys[ct], 0, 0, 0);
             }
         }
         // contraction: slot r of tile ct is Y[i0 + 16 wave + 4 lq + r][t0 + 16 ct + li]
@@ -146,11 +166,10 @@ __global__ void __launch_bounds__(256) k_timesum_mfma(int n, int ntimes, int ldt
         if (lq == 0) red[wave][16 * ct + li] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kTT) {
-        const int t = t0 + threadIdx.x;
+    if (tid < kTT) {
+        const int t = t0 + tid;
         if (t < ntimes) {
-            const double P = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                             red[3][threadIdx.x];
+            const double P = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
             out[(size_t)b * ntimes + t] = (w ? w[b] : 1.0) * P;
         }
     }
