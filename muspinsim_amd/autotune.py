@@ -27,8 +27,6 @@ import json
 import os
 import time
 
-import numpy as np
-
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PATH = os.path.join(HERE, "tuning.json")
 

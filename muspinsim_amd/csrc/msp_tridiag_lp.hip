@@ -1,5 +1,5 @@
 // Panel-blocked Householder tridiagonalisation that reads and writes only the
-// lower triangle (gfx950, 256 threads = 4 waves per matrix).
+// lower triangle (gfx950, one workgroup of W = 4 or 8 waves per matrix).
 //
 // Same reduction as tridiag_team in msp_tridiag.h (A = Q T Q^H, reflector
 // tails stored below the subdiagonal, d/e/tau out), organised for HBM:
@@ -19,7 +19,7 @@
 
 namespace {
 
-constexpr int kWaves = 4;
+constexpr int kCpRows = 4;      // LDS rows of per-wave column partials
 constexpr int kPanelRows = 8;   // rows per batched pass of the end-of-panel update
 #ifndef MSP_TD_RB
 #define MSP_TD_RB 4     // rows per batched pass of the A22 v product (64-lane rows)
@@ -29,6 +29,12 @@ constexpr int kPanelRows = 8;   // rows per batched pass of the end-of-panel upd
 #endif
 #ifndef MSP_TD_RP
 #define MSP_TD_RP 2     // rows per 16-lane group in one pass of that product
+#endif
+#ifndef MSP_TD_WL
+#define MSP_TD_WL 8     // waves per workgroup above n = 192
+#endif
+#ifndef MSP_TD_RB1024
+#define MSP_TD_RB1024 1 // rows per pass of the A22 v product for n <= 1024 (2 spills at 8 waves)
 #endif
 
 // reductions across a wave run on DPP lane moves (msp_team.h)
@@ -44,17 +50,17 @@ __device__ inline double row16_sum(double v)
     return v;
 }
 
-// One pass of the 16-lane-row A22 v product over rows ib + g + 16 r
+// One pass of the 16-lane-row A22 v product over rows ib + g + 4 W r
 // (r < RP), with the first QA column slots (QA is a compile-time constant so
 // that every load of the pass is issued before the first use).
-template <int Q16, int QA, int RP>
+template <int W, int Q16, int QA, int RP>
 __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cdbl *A,
                                    const cdbl *vb, cdbl *pb, cdbl *nxt, cdbl (&colacc)[Q16])
 {
     cdbl av[RP][QA];
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
-        const int i = ib + g + 16 * r;
+        const int i = ib + g + 4 * W * r;
         const int ic = (i < n) ? i : n - 1;
         const cdbl *row = A + (size_t)ic * n;
 #pragma unroll
@@ -65,7 +71,7 @@ __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cd
     }
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
-        const int i = ib + g + 16 * r;
+        const int i = ib + g + 4 * W * r;
         const bool rowok = i < n;
         const cdbl vi = vb[rowok ? i : n - 1];
         double rr = 0.0, ri = 0.0;
@@ -87,13 +93,36 @@ __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cd
     }
 }
 
+// Column partials of one step, every thread of the workgroup calling: wave w
+// stores its per-lane sums acc[q] (column c0 + CS q, lanes with `writer`) in
+// LDS row w % 4 of cp; with W = 8 the waves w and w + 4 share a row, the
+// second adding to the first after a barrier, so the order of the sum is
+// fixed.  (The caller's barrier ends the last phase.)
+template <int W, int Q, int CS>
+__device__ inline void store_colpart(int n, int c0, int wave, bool writer, const cdbl (&acc)[Q],
+                                     cdbl *cp)
+{
+    static_assert(W % kCpRows == 0, "W is a multiple of the partial rows");
+    for (int ph = 0; ph < W / kCpRows; ++ph) {
+        if (ph > 0) __syncthreads();
+        if (wave / kCpRows == ph && writer) {
+            cdbl *row = cp + (size_t)(wave % kCpRows) * n;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int c = c0 + CS * q;
+                if (c < n) row[c] = (ph > 0) ? cadd(row[c], acc[q]) : acc[q];
+            }
+        }
+    }
+}
+
 // A22 v for step k with one matrix row per 16-lane group: wave w takes rows
-// k+1 + 16 t + 4 w + g (g = lane / 16), a lane the columns k+1 + (lane % 16)
+// k+1 + 4 W t + 4 w + g (g = lane / 16), a lane the columns k+1 + (lane % 16)
 // + 16 q.  A row sum is a 16-lane DPP reduction shared by the wave's four
 // rows; the column sums conj(a_ic) v_i stay in registers and are combined
 // across the four groups once per step.  pb[i] receives the row sums, nxt[i]
-// the column k+1 (next step's column), cp[wave][c] the wave's column sums.
-template <int Q16>
+// the column k+1 (next step's column), cp the waves' column sums.
+template <int W, int Q16>
 __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdbl *A, const cdbl *vb,
                                      cdbl *pb, cdbl *nxt, cdbl *cp)
 {
@@ -103,15 +132,15 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
 #pragma unroll
     for (int q = 0; q < Q16; ++q) colacc[q] = cmk(0.0, 0.0);
     constexpr int RP = MSP_TD_RP;
-    for (int ib = k + 1 + 4 * wave; ib < n; ib += 16 * RP) {
-        const int ilast0 = ib + 16 * (RP - 1) + 3;
+    for (int ib = k + 1 + 4 * wave; ib < n; ib += 4 * W * RP) {
+        const int ilast0 = ib + 4 * W * (RP - 1) + 3;
         const int ilast = (ilast0 < n) ? ilast0 : n - 1;
         const int qact = ((ilast - (k + 1)) >> 4) + 1;          // wave-uniform, <= Q16
         // slot counts in steps of 4 keep the number of code paths small
-        if (qact <= 4) rows16_pass<Q16, 4, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        if (qact <= 4) rows16_pass<W, Q16, 4, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
         else if (Q16 >= 8 && qact <= 8)
-            rows16_pass<Q16, (Q16 >= 8 ? 8 : 4), RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
-        else rows16_pass<Q16, Q16, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+            rows16_pass<W, Q16, (Q16 >= 8 ? 8 : 4), RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        else rows16_pass<W, Q16, Q16, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
     }
     const int qstep = ((n - 1 - (k + 1)) >> 4) + 1;
 #pragma unroll
@@ -122,19 +151,19 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
             t.im += __shfl_xor(t.im, 16, 64);
             t.re += __shfl_xor(t.re, 32, 64);
             t.im += __shfl_xor(t.im, 32, 64);
-            const int c = k + 1 + li + 16 * q;
-            if (g == 0 && c < n) cp[wave * n + c] = t;
+            colacc[q] = t;
         }
     }
+    store_colpart<W, Q16, 16>(n, k + 1 + li, wave, g == 0, colacc, cp);
 }
 
 // End-of-panel rank-2k update with lanes owning columns: for each 64-column
 // block the lane's column of Vp and Wp stays in registers while the wave walks
-// the rows at or below the block (rows wave, wave + 4, ..., RB per pass, all
+// the rows at or below the block (rows wave, wave + W, ..., RB per pass, all
 // loads of a pass issued first); a row's Vp and Wp entries are wave-uniform
 // LDS reads.  A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl), c <= i.
 // (always_inline: as a call, the ABI's saved registers spill to scratch)
-template <int NB, int RB>
+template <int W, int NB, int RB>
 __device__ __attribute__((always_inline)) inline void panel_update_cols(
     int n, int t0, int wave, int lane, cdbl *A, const cdbl *Vp, const cdbl *Wp, cdbl *col)
 {
@@ -149,17 +178,17 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
             vc[l] = cconj(Vp[cc * LDP + l]);
             wc[l] = cconj(Wp[cc * LDP + l]);
         }
-        for (int i0 = cb + wave; i0 < n; i0 += RB * kWaves) {
+        for (int i0 = cb + wave; i0 < n; i0 += RB * W) {
             cdbl av[RB];
 #pragma unroll
             for (int u = 0; u < RB; ++u) {
-                const int i = i0 + u * kWaves;
+                const int i = i0 + u * W;
                 const int ic = (i < n) ? i : n - 1;
                 av[u] = A[(size_t)ic * n + ((cc <= ic) ? cc : ic)];
             }
 #pragma unroll
             for (int u = 0; u < RB; ++u) {
-                const int i = i0 + u * kWaves;
+                const int i = i0 + u * W;
                 if (i >= n) break;                       // wave-uniform
                 cdbl a = av[u];
 #pragma unroll
@@ -176,9 +205,11 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
     }
 }
 
-template <int NB, int QMAX, int RB, int Q16, bool GP>
-__global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E, double *work)
+template <int W, int NB, int QMAX, int RB, int Q16, bool GP>
+__global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double *E, double *work)
 {
+    static_assert(W == 4 || W == 8, "4 or 8 waves per matrix");
+    constexpr int kThreads = 64 * W;
     extern __shared__ __align__(16) unsigned char smem[];
     const int b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -195,18 +226,18 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
     cdbl *Wp = Vp + (size_t)n * LDP;                                     // n x LDP
     cdbl *vb = GP ? (cdbl *)smem : Wp + (size_t)n * LDP;                // n
     cdbl *pb = vb + n;                               // n (row sums, then p)
-    cdbl *cp = pb + n;                               // kWaves x n column partials
-    cdbl *y = cp + (size_t)kWaves * n;               // 2 NB
+    cdbl *cp = pb + n;                               // kCpRows x n column partials
+    cdbl *y = cp + (size_t)kCpRows * n;              // 2 NB
     cdbl *colA = y + 2 * NB;                         // n: current column k
     cdbl *colB = colA + n;                           // n: column k+1, stashed by the matvec
-    double *red = (double *)(colB + n);              // 16
+    double *red = (double *)(colB + n);              // 2 W
 
     cdbl *col = colA, *nxt = colB;
-    for (int i = tid; i < n; i += 256) col[i] = A[(size_t)i * n];
+    for (int i = tid; i < n; i += kThreads) col[i] = A[(size_t)i * n];
     for (int k0 = 0; k0 < n - 1; k0 += NB) {
         const int nbk = (n - 1 - k0 < NB) ? n - 1 - k0 : NB;
         // (rows above the panel are never read)
-        for (int idx = k0 * LDP + tid; idx < n * LDP; idx += 256) {
+        for (int idx = k0 * LDP + tid; idx < n * LDP; idx += kThreads) {
             Vp[idx] = cmk(0.0, 0.0);
             Wp[idx] = cmk(0.0, 0.0);
         }
@@ -218,7 +249,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
             // with the squared norm of rows k+2.. summed on the way
             double part = 0.0;
             if (j > 0) {
-                for (int i = k + tid; i < n; i += 256) {
+                for (int i = k + tid; i < n; i += kThreads) {
                     cdbl a = col[i];
                     for (int l = 0; l < j; ++l) {
                         a = csub(a, cmul(Vp[i * LDP + l], cconj(Wp[k * LDP + l])));
@@ -228,13 +259,14 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                     if (i >= k + 2) part += cabs2(a);
                 }
             } else {
-                for (int i = k + 2 + tid; i < n; i += 256) part += cabs2(col[i]);
+                for (int i = k + 2 + tid; i < n; i += kThreads) part += cabs2(col[i]);
             }
             // (b) reflector of A[k+1.., k]
             part = wave_sum(part);
             if (lane == 0) red[wave] = part;
             __syncthreads();
-            const double xn2 = red[0] + red[1] + red[2] + red[3];
+            double xn2 = red[0];
+            for (int w = 1; w < W; ++w) xn2 += red[w];
             const cdbl alpha = col[k + 1];
             double beta;
             cdbl tk, scale;
@@ -249,7 +281,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 scale = cdivc(cmk(1.0, 0.0), csub(alpha, cmk(beta, 0.0)));
             }
             const bool act = (tk.re != 0.0 || tk.im != 0.0);
-            for (int i = k + 1 + tid; i < n; i += 256) {
+            for (int i = k + 1 + tid; i < n; i += kThreads) {
                 cdbl vi;
                 if (i == k + 1) vi = cmk(1.0, 0.0);
                 else vi = cmul(col[i], scale);
@@ -259,7 +291,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
             __syncthreads();
             if (act) {
                 // (c) y[l] = Wp(:, l)^H v, y[j + l] = Vp(:, l)^H v over rows k+1..
-                for (int l = wave; l < 2 * j; l += kWaves) {
+                for (int l = wave; l < 2 * j; l += W) {
                     const cdbl *M = (l < j) ? Wp : Vp;
                     const int c = (l < j) ? l : l - j;
                     double sr = 0.0, si = 0.0;
@@ -274,19 +306,19 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 }
                 // (d) A22 v over the lower triangle of rows/cols k+1..n-1
                 if constexpr (Q16 > 0) {
-                    matvec_rows16<Q16>(n, k, wave, lane, A, vb, pb, nxt, cp);
+                    matvec_rows16<W, Q16>(n, k, wave, lane, A, vb, pb, nxt, cp);
                 } else {
                     cdbl colacc[QMAX];
     #pragma unroll
                     for (int q = 0; q < QMAX; ++q) colacc[q] = cmk(0.0, 0.0);
-                    // RB rows per pass (i, i+4, ... of this wave): every load of
+                    // RB rows per pass (i, i+W, ... of this wave): every load of
                     // the pass is issued first, from clamped addresses with the
                     // value masked, so no branch separates the loads
-                    for (int i0 = k + 1 + wave; i0 < n; i0 += RB * kWaves) {
+                    for (int i0 = k + 1 + wave; i0 < n; i0 += RB * W) {
                         cdbl av[RB][QMAX];
     #pragma unroll
                         for (int u = 0; u < RB; ++u) {
-                            const int i = i0 + u * kWaves;
+                            const int i = i0 + u * W;
                             const int ic = (i < n) ? i : n - 1;
                             const cdbl *row = A + (size_t)ic * n;
     #pragma unroll
@@ -299,7 +331,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                         double rr[RB], ri[RB];
     #pragma unroll
                         for (int u = 0; u < RB; ++u) {
-                            const int i = i0 + u * kWaves;
+                            const int i = i0 + u * W;
                             const bool rowok = i < n;
                             const cdbl vi = vb[rowok ? i : n - 1];
                             rr[u] = 0.0;
@@ -325,23 +357,19 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                         if (lane == 63) {
     #pragma unroll
                             for (int u = 0; u < RB; ++u) {
-                                const int i = i0 + u * kWaves;
+                                const int i = i0 + u * W;
                                 if (i < n) pb[i] = cmk(rr[u], ri[u]);
                             }
                         }
                     }
-    #pragma unroll
-                    for (int q = 0; q < QMAX; ++q) {
-                        const int c = k + 1 + lane + 64 * q;
-                        if (c < n) cp[wave * n + c] = colacc[q];
-                    }
+                    store_colpart<W, QMAX, 64>(n, k + 1 + lane, wave, true, colacc, cp);
                 }
                 __syncthreads();
                 // (e) p = tau (A22 v - Vp y1 - Wp y2), and p^H v
                 double pre = 0.0, pim = 0.0;
-                for (int i = k + 1 + tid; i < n; i += 256) {
+                for (int i = k + 1 + tid; i < n; i += kThreads) {
                     cdbl acc = pb[i];
-                    for (int w = 0; w < kWaves; ++w) acc = cadd(acc, cp[w * n + i]);
+                    for (int w = 0; w < kCpRows; ++w) acc = cadd(acc, cp[w * n + i]);
                     for (int l = 0; l < j; ++l) {
                         acc = csub(acc, cmul(Vp[i * LDP + l], y[l]));
                         acc = csub(acc, cmul(Wp[i * LDP + l], y[j + l]));
@@ -358,17 +386,20 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
                 // v write, so it can be overwritten here)
                 if (lane == 0) {
                     red[wave] = pre;
-                    red[4 + wave] = pim;
+                    red[W + wave] = pim;
                 }
                 __syncthreads();
-                const double sre = red[0] + red[1] + red[2] + red[3];
-                const double sim = red[4] + red[5] + red[6] + red[7];
+                double sre = red[0], sim = red[W];
+                for (int w = 1; w < W; ++w) {
+                    sre += red[w];
+                    sim += red[W + w];
+                }
                 const cdbl a2 = cmul(cmk(-0.5 * tk.re, -0.5 * tk.im), cmk(sre, sim));
                 // (f) w = p + a2 v
-                for (int i = k + 1 + tid; i < n; i += 256) Wp[i * LDP + j] = cadd(pb[i], cmul(a2, vb[i]));
+                for (int i = k + 1 + tid; i < n; i += kThreads) Wp[i * LDP + j] = cadd(pb[i], cmul(a2, vb[i]));
             }
             if (!act)
-                for (int i = k + 1 + tid; i < n; i += 256) nxt[i] = A[(size_t)i * n + k + 1];
+                for (int i = k + 1 + tid; i < n; i += kThreads) nxt[i] = A[(size_t)i * n + k + 1];
             if (tid == 0) {
                 d[k] = col[k].re;
                 e[k] = beta;
@@ -385,11 +416,11 @@ __global__ void __launch_bounds__(256) k_tridiag_lp(int n, double *H, double *E,
         // end of panel: reflector tails of the panel's columns go to memory
         // (rows k+2.., below the subdiagonal), and the trailing lower triangle
         // takes A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl)
-        for (int idx = tid; idx < n * nbk; idx += 256) {
+        for (int idx = tid; idx < n * nbk; idx += kThreads) {
             const int i = idx / nbk, l = idx % nbk;
             if (i >= k0 + l + 2) A[(size_t)i * n + k0 + l] = Vp[i * LDP + l];
         }
-        panel_update_cols<NB, kPanelRows>(n, k0 + nbk, wave, lane, A, Vp, Wp, col);
+        panel_update_cols<W, NB, kPanelRows>(n, k0 + nbk, wave, lane, A, Vp, Wp, col);
         __syncthreads();
     }
     if (tid == 0) d[n - 1] = col[n - 1].re;
@@ -399,18 +430,18 @@ template <int NB, bool GP>
 size_t lp_lds(int n)
 {
     return sizeof(cdbl) * ((GP ? 0 : (size_t)2 * n * (NB + 1)) + 4 * (size_t)n +
-                           (size_t)kWaves * n + 2 * NB) +
+                           (size_t)kCpRows * n + 2 * NB) +
            16 * sizeof(double);
 }
 
-template <int NB, int QMAX, int RB, int Q16, bool GP = false>
+template <int W, int NB, int QMAX, int RB, int Q16, bool GP = false>
 hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
     size_t lds = lp_lds<NB, GP>(n);
-    hipError_t err = msp_set_lds((const void *)k_tridiag_lp<NB, QMAX, RB, Q16, GP>, lds);
+    hipError_t err = msp_set_lds((const void *)k_tridiag_lp<W, NB, QMAX, RB, Q16, GP>, lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((k_tridiag_lp<NB, QMAX, RB, Q16, GP>), dim3(batch), dim3(256), lds, s, n, H,
-                       E, work);
+    hipLaunchKernelGGL((k_tridiag_lp<W, NB, QMAX, RB, Q16, GP>), dim3(batch), dim3(64 * W), lds, s, n,
+                       H, E, work);
     return hipGetLastError();
 }
 
@@ -420,11 +451,12 @@ hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipSt
 // n = 512); returns hipErrorNotSupported above.
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
-    if (n <= 64) return lp_launch<8, 1, MSP_TD_RB, MSP_TD_ROWS16 ? 4 : 0>(n, batch, H, E, work, s);
-    if (n <= 128) return lp_launch<8, 2, MSP_TD_RB, MSP_TD_ROWS16 ? 8 : 0>(n, batch, H, E, work, s);
-    if (n <= 192) return lp_launch<8, 3, MSP_TD_RB, MSP_TD_ROWS16 ? 12 : 0>(n, batch, H, E, work, s);
-    if (n <= 256) return lp_launch<8, 4, 4, 0>(n, batch, H, E, work, s);
-    if (n <= 512) return lp_launch<4, 8, 4, 0>(n, batch, H, E, work, s);
-    if (n <= 1024) return lp_launch<4, 16, 2, 0, true>(n, batch, H, E, work, s);
+    if (n <= 64) return lp_launch<4, 8, 1, MSP_TD_RB, MSP_TD_ROWS16 ? 4 : 0>(n, batch, H, E, work, s);
+    if (n <= 128) return lp_launch<4, 8, 2, MSP_TD_RB, MSP_TD_ROWS16 ? 8 : 0>(n, batch, H, E, work, s);
+    if (n <= 192) return lp_launch<4, 8, 3, MSP_TD_RB, MSP_TD_ROWS16 ? 12 : 0>(n, batch, H, E, work, s);
+    // above n = 192 one workgroup fills a CU's LDS: 8 waves put two on each SIMD
+    if (n <= 256) return lp_launch<MSP_TD_WL, 8, 4, 4, 0>(n, batch, H, E, work, s);
+    if (n <= 512) return lp_launch<MSP_TD_WL, 4, 8, 4, 0>(n, batch, H, E, work, s);
+    if (n <= 1024) return lp_launch<MSP_TD_WL, 4, 16, MSP_TD_RB1024, 0, true>(n, batch, H, E, work, s);
     return hipErrorNotSupported;
 }
