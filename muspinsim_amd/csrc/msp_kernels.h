@@ -82,6 +82,10 @@ hipError_t launch_accumulate(int nslots, int width, const int32_t *ptr, const in
 void prof_enable(int on);
 std::string prof_report();
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s);
+// Register-resident reduction (matrix in VGPRs, one workgroup of 8 waves per
+// matrix) for 2 <= n <= 64, same outputs as launch_tridiag_lp.
+size_t tridiag_rs_lds_bytes(int n);
+hipError_t launch_tridiag_rs(int n, int batch, double *H, double *E, double *work, hipStream_t s);
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
                                      hipStream_t s);
 

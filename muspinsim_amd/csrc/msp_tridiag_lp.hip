@@ -451,6 +451,8 @@ hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipSt
 // n = 512); returns hipErrorNotSupported above.
 hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
+    // n <= 64: the register-resident kernel (msp_tridiag_rs.hip) is faster
+    if (n >= 2 && n <= 64) return launch_tridiag_rs(n, batch, H, E, work, s);
     if (n <= 64) return lp_launch<4, 8, 1, MSP_TD_RB, MSP_TD_ROWS16 ? 4 : 0>(n, batch, H, E, work, s);
     if (n <= 128) return lp_launch<4, 8, 2, MSP_TD_RB, MSP_TD_ROWS16 ? 8 : 0>(n, batch, H, E, work, s);
     if (n <= 192) return lp_launch<4, 8, 3, MSP_TD_RB, MSP_TD_ROWS16 ? 12 : 0>(n, batch, H, E, work, s);
