@@ -81,22 +81,40 @@ def time_eigh(lib, n, batch, method, reps=3):
     return max(_events_ms(call, reps) - copy_ms, 1e-4)
 
 
-def tune_eigh(lib, ns=EIGH_NS, batches=EIGH_BATCHES, reps=3):
-    """{n: [[batch_min, method], ...]}: the faster eigensolver from each
-    batch size on (methods 'jacobi' / 'dc'), and the raw timings."""
-    from . import _native as N
-    table, raw = {}, {}
-    for n in ns:
+def eigh_ladders(raw, tol=0.05):
+    """{n: [[batch_min, method], ...]} from timings {"<n>x<batch>": {"jacobi_ms",
+    "dc_ms"}}: the faster eigensolver from each batch size on, switching only
+    when the other method is faster by more than `tol` (differences within the
+    run-to-run noise keep the previous choice)."""
+    by_n = {}
+    for key, t in raw.items():
+        n, batch = (int(x) for x in key.split("x"))
+        by_n.setdefault(n, []).append((batch, t["jacobi_ms"], t["dc_ms"]))
+    table = {}
+    for n, rows in sorted(by_n.items()):
         ladder = []
+        for batch, tj, td in sorted(rows):
+            if not ladder:
+                ladder.append([1, "jacobi" if tj <= td else "dc"])
+                continue
+            cur = ladder[-1][1]
+            t_cur, t_other = (tj, td) if cur == "jacobi" else (td, tj)
+            if t_other < t_cur * (1.0 - tol):
+                ladder.append([batch, "dc" if cur == "jacobi" else "jacobi"])
+        table[str(n)] = ladder
+    return table
+
+
+def tune_eigh(lib, ns=EIGH_NS, batches=EIGH_BATCHES, reps=3):
+    """Eigensolver ladders per dimension (eigh_ladders) and the raw timings."""
+    from . import _native as N
+    raw = {}
+    for n in ns:
         for batch in batches:
             tj = time_eigh(lib, n, batch, N.MSP_EIGH_JACOBI, reps)
             td = time_eigh(lib, n, batch, N.MSP_EIGH_DC, reps)
             raw[f"{n}x{batch}"] = {"jacobi_ms": tj, "dc_ms": td}
-            best = "jacobi" if tj <= td else "dc"
-            if not ladder or ladder[-1][1] != best:
-                ladder.append([batch if ladder else 1, best])
-        table[str(n)] = ladder
-    return table, raw
+    return eigh_ladders(raw), raw
 
 
 def tune_timesum(lib, ns=TIMESUM_NS, ntimes=2000, batch=256, reps=3):

@@ -76,3 +76,19 @@ def test_autotune_ladder_and_engine_choice():
     assert autotune.eigh_method(t, 32, 10) == N.MSP_EIGH_AUTO
     assert autotune.eigh_method(None, 64, 10) == N.MSP_EIGH_AUTO
     assert autotune.load("/nonexistent/tuning.json") is None
+
+
+def test_autotune_ladder_hysteresis():
+    """Ladders from raw timings: the faster method from each batch on, but a
+    difference within the tolerance keeps the previous choice."""
+    from muspinsim_amd import autotune
+    raw = {"12x64": {"jacobi_ms": 0.116, "dc_ms": 0.118},
+           "12x256": {"jacobi_ms": 0.121, "dc_ms": 0.119},     # 1.7 % apart: stays Jacobi
+           "12x1024": {"jacobi_ms": 0.137, "dc_ms": 0.187},
+           "16x64": {"jacobi_ms": 0.165, "dc_ms": 0.135},
+           "16x1024": {"jacobi_ms": 0.188, "dc_ms": 0.230},    # Jacobi 18 % faster: switch
+           "16x4096": {"jacobi_ms": 0.401, "dc_ms": 0.390}}    # 2.7 % apart: stays Jacobi
+    t = autotune.eigh_ladders(raw, tol=0.05)
+    assert t["12"] == [[1, "jacobi"]]
+    assert t["16"] == [[1, "dc"], [1024, "jacobi"]]
+    assert autotune.eigh_ladders(raw, tol=0.0)["12"] == [[1, "jacobi"], [256, "dc"], [1024, "jacobi"]]
