@@ -313,8 +313,11 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 // the merges live in LDS when `small_in_lds` (dynamic LDS holds them), the
 // n x n matrices (Q0, Q1, S, QK) in the global workspace.
 // (four waves per SIMD: the compiler fits the kernel in 128 registers
-// without spilling, so four instances share a CU instead of three)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// without spilling, so four 256-thread instances share a CU instead of three.
+// NT = 1024 puts one instance on a whole CU, for the large dimensions whose
+// instances would otherwise leave one wave per SIMD.)
+template <int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_stedc(int n, double *E, double *work, int32_t *info, int small_in_lds)
 {
     __shared__ double red[256];
@@ -1024,10 +1027,18 @@ static hipError_t eigh_dc_middle(int n, int batch, double *H, double *E, int32_t
     if (err == hipSuccess) {
         const int in_lds = (n <= 512) ? 1 : 0;
         const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
-        err = msp_set_lds((const void *)k_stedc, lds2);
+        // one 1024-thread workgroup per instance where the instance's LDS
+        // arrays leave room for one workgroup per CU (n >= 466) or live in
+        // global memory (n > 512): 256 threads would leave one wave per SIMD
+        // (n = 512: 9.8 -> 5.2 ms, n = 1024: 64 -> 28 ms per 140 instances)
+        const bool big = !in_lds || 2 * lds2 > 160 * 1024;
+        err = msp_set_lds(big ? (const void *)k_stedc<1024> : (const void *)k_stedc<256>, lds2);
         if (err == hipSuccess) {
             ProfScope ps_("k_stedc", s);
-            hipLaunchKernelGGL(k_stedc, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
+            if (big)
+                hipLaunchKernelGGL(k_stedc<1024>, dim3(batch), dim3(1024), lds2, s, n, E, work, info, in_lds);
+            else
+                hipLaunchKernelGGL(k_stedc<256>, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
             err = hipGetLastError();
         }
     }
