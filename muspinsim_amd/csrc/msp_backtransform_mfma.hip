@@ -13,7 +13,8 @@
 //   U -= (V T) W    in the accumulators of every tile below the block; each
 //                   wave sums the W partials itself, V T comes from the WY
 //                   factor kernel
-// One barrier per block (the W partials alternate between two LDS buffers).
+// One barrier per block (the W partials alternate between two LDS buffers;
+// with 8 waves a second one, after the partials are summed once per element).
 // U never leaves registers, so LDS holds only the W partials, and several
 // workgroups share a CU.
 #include <hip/hip_runtime.h>
@@ -45,6 +46,7 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
                                                                double *U)
 {
     __shared__ double Wp[2][NW][2][256];      // per-wave partial W (re, im), by block parity
+    __shared__ double Ws[(NW > 4) ? 512 : 1];  // NW > 4: W summed over the waves (re, im)
     int chunk, b;
     msp_xcd_tile(gridDim.x, gridDim.y, chunk, b);
     const int c0 = chunk * CW;
@@ -100,17 +102,37 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
         __syncthreads();
         // ---- U -= (V T) W in the accumulators of the tiles below the block
         double br[4], bi[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int o = (4 * s + lq) * 16 + li;
-            double sr = 0.0, si = 0.0;
-#pragma unroll
-            for (int w2 = 0; w2 < NW; ++w2) {
-                sr += Wp[par][w2][0][o];
-                si += Wp[par][w2][1][o];
+        if constexpr (NW > 4) {
+            // one thread per element of W sums the wave partials (in wave
+            // order) and every lane then reads its four: summing in every
+            // lane would hold 8 NW loads in flight, which the 8-wave form
+            // has no registers for (n = 1024: 54.9 -> 45.1 ms per 140)
+            if (tid < 512) {
+                const int part = tid >> 8, o = tid & 255;
+                double sum = 0.0;
+                for (int w2 = 0; w2 < NW; ++w2) sum += Wp[par][w2][part][o];
+                Ws[tid] = sum;
             }
-            br[s] = sr;
-            bi[s] = si;
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int o = (4 * s + lq) * 16 + li;
+                br[s] = Ws[o];
+                bi[s] = Ws[256 + o];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int o = (4 * s + lq) * 16 + li;
+                double sr = 0.0, si = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < NW; ++w2) {
+                    sr += Wp[par][w2][0][o];
+                    si += Wp[par][w2][1][o];
+                }
+                br[s] = sr;
+                bi[s] = si;
+            }
         }
         const cdbl *VT = VTall + (size_t)blk * MSP_BT_NB * n;
 #pragma unroll
