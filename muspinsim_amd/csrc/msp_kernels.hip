@@ -813,15 +813,45 @@ __global__ void __launch_bounds__(256) k_integral_obs(int n, const double *E, co
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_accumulate(int nslots, int width, const int32_t *ptr, const int32_t *idx,
-                             const double *rows, double scale, double *results)
+// results[s][t] += scale * sum of rows[idx[k]][t] over the instances k of
+// slot s.  A workgroup owns 64 output elements; its kAccParts waves sum
+// contiguous parts of each slot's instance range (four loads in flight per
+// lane) and wave 0 adds the parts in order, so the sum order is fixed.
+constexpr int kAccParts = 4;
+
+__global__ void __launch_bounds__(64 * kAccParts) k_accumulate(int nslots, int width, const int32_t *ptr,
+                                                               const int32_t *idx, const double *rows,
+                                                               double scale, double *results)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nslots * width) return;
-    const int s = g / width, t = g % width;
-    double acc = results[g];
-    for (int k = ptr[s]; k < ptr[s + 1]; ++k) acc += rows[(size_t)idx[k] * width + t] * scale;
-    results[g] = acc;
+    __shared__ double part[kAccParts][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long g = (long long)blockIdx.x * 64 + lane;
+    const bool ok = g < (long long)nslots * width;
+    double acc = 0.0;
+    if (ok) {
+        const int s = (int)(g / width), t = (int)(g % width);
+        const int k0 = ptr[s], len = ptr[s + 1] - k0;
+        const int a = k0 + (int)((long long)len * wave / kAccParts);
+        const int b = k0 + (int)((long long)len * (wave + 1) / kAccParts);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = a;
+        for (; k + 3 < b; k += 4) {
+            a0 += rows[(size_t)idx[k] * width + t];
+            a1 += rows[(size_t)idx[k + 1] * width + t];
+            a2 += rows[(size_t)idx[k + 2] * width + t];
+            a3 += rows[(size_t)idx[k + 3] * width + t];
+        }
+        for (; k < b; ++k) a0 += rows[(size_t)idx[k] * width + t];
+        acc = (a0 + a1) + (a2 + a3);
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && ok) {
+        double sum = part[0][lane];
+#pragma unroll
+        for (int w = 1; w < kAccParts; ++w) sum += part[w][lane];
+        results[g] += scale * sum;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,8 +1239,8 @@ hipError_t launch_accumulate(int nslots, int width, const int32_t *ptr, const in
                              const double *rows, double scale, double *results, hipStream_t s)
 {
     { ProfScope ps_("k_accumulate", s);
-    hipLaunchKernelGGL(k_accumulate, dim3(cdiv((size_t)nslots * width, 256)), dim3(256), 0, s,
-                       nslots, width, ptr, idx, rows, scale, results);
+    hipLaunchKernelGGL(k_accumulate, dim3(cdiv((size_t)nslots * width, 64)), dim3(64 * kAccParts), 0,
+                       s, nslots, width, ptr, idx, rows, scale, results);
     }
     return hipGetLastError();
 }
