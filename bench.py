@@ -49,8 +49,8 @@ def algorithmic(kernel, n, ntimes, real_mode=True):
         return "flop", 4.0 / 3.0 * n ** 3
     if kernel == "k_timesum":
         return "flop", (4.0 if real_mode else 8.0) * n * n * ntimes
-    if kernel == "k_timesum_mfma":
-        # Y = C [cos | sin]: n x n x 2T real multiply-adds
+    if kernel in ("k_timesum_mfma", "k_timesum_mfma64"):
+        # Y = C [cos | sin]: n x n x 2T real multiply-adds (FP32 / FP64)
         return "flop", 4.0 * n * n * ntimes
     if kernel == "k_tridiag32":
         return "flop", 16.0 / 3.0 * n ** 3

@@ -225,6 +225,17 @@ int msp_timesum_mixed(int n, int batch, int ntimes, const double *d_E, const dou
                       const double *d_times, const double *d_w, double *d_out, void *d_work,
                       size_t work_bytes, void *stream);
 
+/* msp_timesum for a real symmetric C (real_mode = 1, the same routes and
+ * reference lines as msp_timesum_mixed) in FP64 on matrix cores:
+ * d_out[b*T + t] = w_b sum_ij C_ij cos(2 pi (E_i - E_j) t), the phases formed
+ * exactly as msp_timesum forms them but once per (j, t), the sum over j on
+ * FP64 matrix cores.  Agrees with msp_timesum to FP64 rounding.  d_work holds
+ * the FP64 phase table (msp_timesum_mfma64_workspace bytes); batch <= 65535. */
+size_t msp_timesum_mfma64_workspace(int n, int batch, int ntimes);
+int msp_timesum_mfma64(int n, int batch, int ntimes, const double *d_E, const double *d_C,
+                       const double *d_times, const double *d_w, double *d_out, void *d_work,
+                       size_t work_bytes, void *stream);
+
 /* I_b = sum_ij C_ij / (1/tau + 2 pi i (E_i - E_j)) * out_scale.
  * out_complex = 0: d_out[b] = w_b Re I_b; 1: complex, unweighted. */
 int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode,

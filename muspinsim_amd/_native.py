@@ -59,6 +59,8 @@ SIGNATURES = {
     "msp_eigh_mixed_workspace": (_SZ, [_I, _I]),
     "msp_timesum_mixed_workspace": (_SZ, [_I, _I, _I]),
     "msp_timesum_mixed": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _SZ, _P]),
+    "msp_timesum_mfma64_workspace": (_SZ, [_I, _I, _I]),
+    "msp_timesum_mfma64": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _SZ, _P]),
     "msp_eigh_mixed": (_I, [_I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "msp_apply_muon_op": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "msp_apply_kron": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P]),

@@ -12,7 +12,9 @@ choice is measured, not assumed:
                 solve, so it wins on latency for small batches and loses on
                 throughput for large ones.
   timesum       the smallest n from which the mixed-precision time sum
-                (FP32 matrix cores) beats the FP64 one on the same batch.
+                (FP32 matrix cores) beats the FP64 one on the same batch, and
+                the smallest n from which the FP64 matrix-core time sum beats
+                the FP64 VALU one.
   max_chunk     instances per launch of the n = 192 divide-and-conquer path.
 
 `tune()` times them with HIP events around the library calls and returns a
@@ -146,6 +148,49 @@ def tune_timesum(lib, ns=TIMESUM_NS, ntimes=2000, batch=256, reps=3):
     return (first if first is not None else 1 << 30), raw
 
 
+def tune_timesum64(lib, ns=TIMESUM_NS + (192, 256), ntimes=2000, batch=256, reps=3, tol=0.05):
+    """Smallest n (of ns) from which the FP64 matrix-core time sum (phase
+    table + MFMA GEMM) is faster than the FP64 VALU one by more than `tol`
+    at this n and every larger n measured, and the raw timings."""
+    import torch
+    from . import _native as N
+    raw = {}
+    st = torch.cuda.current_stream().cuda_stream
+    t = torch.linspace(0, 10, ntimes, dtype=torch.float64, device="cuda")
+    for n in ns:
+        E = torch.sort(torch.randn((batch, n), dtype=torch.float64, device="cuda") * 50, dim=1).values
+        C = torch.randn((batch, n, n), dtype=torch.float64, device="cuda")
+        C = ((C + C.transpose(1, 2)) / 2).contiguous()
+        out = torch.empty((batch, ntimes), dtype=torch.float64, device="cuda")
+        ws = lib.msp_timesum_mfma64_workspace(n, batch, ntimes)
+        work = torch.empty((max(ws, 8) // 8,), dtype=torch.float64, device="cuda")
+        t64 = _events_ms(lambda: N.check(lib.msp_timesum(n, batch, ntimes, E.data_ptr(), C.data_ptr(),
+                                                         1, t.data_ptr(), None, 0, out.data_ptr(),
+                                                         st)), reps)
+        tm = _events_ms(lambda: N.check(lib.msp_timesum_mfma64(n, batch, ntimes, E.data_ptr(),
+                                                               C.data_ptr(), t.data_ptr(), None,
+                                                               out.data_ptr(), work.data_ptr(), ws,
+                                                               st)), reps)
+        raw[str(n)] = {"valu_ms": t64, "mfma_ms": tm}
+        del E, C, out, work
+        torch.cuda.empty_cache()
+    return timesum64_threshold(raw, tol), raw
+
+
+def timesum64_threshold(raw, tol=0.05):
+    """The smallest n from which every measured n has the matrix-core time
+    sum faster by more than `tol` (a very large n when none does)."""
+    ns = sorted(int(k) for k in raw)
+    first = 1 << 30
+    for n in reversed(ns):
+        r = raw[str(n)]
+        if r["mfma_ms"] < r["valu_ms"] * (1.0 - tol):
+            first = n
+        else:
+            break
+    return first
+
+
 def tune_chunk(candidates=(2048, 4096, 8192), nfields=40, nori=500):
     """Instances per launch for the n = 192 integral sweep: best of one
     timed execute per candidate on a 40-field slice of the sweep."""
@@ -183,13 +228,16 @@ def tune(quick=False):
     batches = (64, 1024, 8192) if quick else EIGH_BATCHES
     eigh, eigh_raw = tune_eigh(lib, batches=batches, reps=reps)
     ts_min, ts_raw = tune_timesum(lib, reps=reps)
+    ts64_min, ts64_raw = tune_timesum64(lib, reps=reps)
     chunk, chunk_raw = tune_chunk() if not quick else (8192, {})
     return {
         "device": torch.cuda.get_device_properties(0).gcnArchName,
         "eigh": eigh,
         "mixed_timesum_min_n": ts_min,
+        "fp64_timesum_mfma_min_n": ts64_min,
         "max_chunk": chunk,
-        "measured": {"eigh_ms": eigh_raw, "timesum_ms": ts_raw, "chunk_ms": chunk_raw},
+        "measured": {"eigh_ms": eigh_raw, "timesum_ms": ts_raw, "timesum64_ms": ts64_raw,
+                     "chunk_ms": chunk_raw},
     }
 
 

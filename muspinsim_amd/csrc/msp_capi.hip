@@ -418,6 +418,31 @@ int msp_timesum_mixed(int n, int batch, int ntimes, const double *d_E, const dou
                       "msp_timesum_mixed");
 }
 
+size_t msp_timesum_mfma64_workspace(int n, int batch, int ntimes)
+{
+    if (n < 1 || batch < 1 || ntimes < 1) return 0;
+    return timesum_mfma64_workspace(n, batch, ntimes);
+}
+
+int msp_timesum_mfma64(int n, int batch, int ntimes, const double *d_E, const double *d_C,
+                       const double *d_times, const double *d_w, double *d_out, void *d_work,
+                       size_t work_bytes, void *stream)
+{
+    if (n < 1 || n > kMaxDim) return fail(MSP_ERR_VALUE, "msp_timesum_mfma64: n out of range");
+    if (batch < 0 || ntimes < 0) return fail(MSP_ERR_VALUE, "msp_timesum_mfma64: negative size");
+    if (batch == 0 || ntimes == 0) return MSP_OK;
+    if (batch > 65535) return fail(MSP_ERR_VALUE, "msp_timesum_mfma64: batch > 65535");
+    if (!d_E || !d_C || !d_times || !d_out)
+        return fail(MSP_ERR_VALUE, "msp_timesum_mfma64: null pointer");
+    const size_t need = timesum_mfma64_workspace(n, batch, ntimes);
+    if (!d_work || work_bytes < need)
+        return fail(MSP_ERR_VALUE, "msp_timesum_mfma64: workspace of %zu bytes needed, %zu given",
+                    need, work_bytes);
+    return hip_status(launch_timesum_mfma64(n, batch, ntimes, d_E, d_C, d_times, d_w, d_out, d_work,
+                                            S(stream)),
+                      "msp_timesum_mfma64");
+}
+
 int msp_integral(int n, int batch, const double *d_E, const double *d_C, int real_mode, double tau,
                  double out_scale, const double *d_w, int out_complex, double *d_out, void *stream)
 {

@@ -53,6 +53,10 @@ hipError_t launch_eigh_from_tridiag(int n, int batch, const double *R, double *E
 // contraction with the phases per 64-time tile.
 int timesum_mfma_ldt(int ntimes);
 size_t timesum_mfma_workspace(int n, int batch, int ntimes);
+size_t timesum_mfma64_workspace(int n, int batch, int ntimes);
+hipError_t launch_timesum_mfma64(int n, int batch, int ntimes, const double *E, const double *C,
+                                 const double *times, const double *w, double *out, void *work,
+                                 hipStream_t s);
 hipError_t launch_timesum_mfma(int n, int batch, int ntimes, const double *E, const double *C,
                                const double *times, const double *w, double *out, void *work,
                                hipStream_t s);

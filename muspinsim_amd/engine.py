@@ -160,6 +160,8 @@ class SpinEngine:
             max_chunk = int(t.get("max_chunk", 8192))
         self.mixed_min_n = int(t.get("mixed_timesum_min_n", 48))
         self.mixed_min_work = float(t.get("mixed_timesum_min_work", 1e9))
+        # FP64 time sum on matrix cores (plain precision) from this n on
+        self.mfma64_min_n = int(t.get("fp64_timesum_mfma_min_n", 48))
         self.precision = precision
         self.group_streams = int(group_streams)
         self.chunk_bytes = chunk_bytes
@@ -437,15 +439,22 @@ class SpinEngine:
         g.work = torch.empty((max(g.wsb, 8) // 8,), dtype=torch.float64, device=dev)
         real = g.route != "general"
         g.C = torch.empty((cs, n, n), dtype=torch.float64 if real else torch.complex128, device=dev)
-        g.tswork, g.tswsb = None, 0
-        # (below mixed_min_n, tuned, 48 without tuning, the time sum takes a
-        # few microseconds either way and the phase-table launch costs more)
-        # and below about 1e9 multiply-adds per launch the phase-table launch
-        # is not repaid either (few instances, short time axes)
-        if (self.precision == "mixed" and real and g.width > 1 and n >= self.mixed_min_n
-                and float(n) * n * g.width * cs >= self.mixed_min_work):
-            g.tswsb = self.lib.msp_timesum_mixed_workspace(n, cs, g.width)
-            g.tswork = torch.empty((max(g.tswsb, 8) // 8,), dtype=torch.float64, device=dev)
+        g.tswork, g.tswsb, g.tsmode = None, 0, None
+        # The real routes' time sum runs on matrix cores from a phase table:
+        # FP32 products for precision="mixed", FP64 for "fp64".  Below the
+        # tuned dimension (48 without tuning) the time sum takes a few
+        # microseconds either way and the phase-table launch costs more, and
+        # below about 1e9 multiply-adds per launch the phase-table launch is
+        # not repaid either (few instances, short time axes).
+        if real and g.width > 1 and float(n) * n * g.width * cs >= self.mixed_min_work:
+            if self.precision == "mixed" and n >= self.mixed_min_n:
+                g.tsmode = "mixed"
+                g.tswsb = self.lib.msp_timesum_mixed_workspace(n, cs, g.width)
+            elif self.precision == "fp64" and n >= self.mfma64_min_n:
+                g.tsmode = "mfma64"
+                g.tswsb = self.lib.msp_timesum_mfma64_workspace(n, cs, g.width)
+            if g.tsmode is not None:
+                g.tswork = torch.empty((max(g.tswsb, 8) // 8,), dtype=torch.float64, device=dev)
         if not real:
             g.R = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
             g.T2 = torch.empty((cs, n, n), dtype=torch.complex128, device=dev)
@@ -594,10 +603,11 @@ class SpinEngine:
             if batch.y == "integral":
                 N.check(L.msp_integral(n, nb, g.E.data_ptr(), g.C.data_ptr(), real, MU_TAU,
                                        1.0 / MU_TAU, p(g.w), 0, out.data_ptr(), st))
-            elif g.tswork is not None:
-                N.check(L.msp_timesum_mixed(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(),
-                                            plan.times.data_ptr(), p(g.w), out.data_ptr(),
-                                            g.tswork.data_ptr(), g.tswsb, st))
+            elif g.tsmode is not None:
+                fn = L.msp_timesum_mixed if g.tsmode == "mixed" else L.msp_timesum_mfma64
+                N.check(fn(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(),
+                           plan.times.data_ptr(), p(g.w), out.data_ptr(),
+                           g.tswork.data_ptr(), g.tswsb, st))
             else:
                 N.check(L.msp_timesum(n, nb, plan.width, g.E.data_ptr(), g.C.data_ptr(), real,
                                       plan.times.data_ptr(), p(g.w), 0, out.data_ptr(), st))
