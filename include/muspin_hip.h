@@ -97,6 +97,8 @@ typedef struct MspTerm {
 #define MSP_EIGH_AUTO 0
 #define MSP_EIGH_JACOBI 1   /* LDS-resident cyclic Jacobi, n <= 64 */
 #define MSP_EIGH_DC 2       /* Householder + divide and conquer, any n */
+#define MSP_EIGH_DC2 3      /* two-stage reduction (band of 16 on FP64 matrix cores,
+                               then bulge chasing) + divide and conquer, 34 <= n <= 256 */
 
 int msp_init(void);
 int msp_last_error(char *h_buf, size_t len);

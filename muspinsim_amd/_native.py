@@ -24,6 +24,7 @@ MSP_ERR_UNSUPPORTED = 4
 MSP_EIGH_AUTO = 0
 MSP_EIGH_JACOBI = 1
 MSP_EIGH_DC = 2
+MSP_EIGH_DC2 = 3
 
 MSP_MAX_SPINS = 16
 MSP_NOALLOC_SCRATCH_BYTES = 64 << 20     # static scratch of msp_eigh_noalloc, per device

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmuspin_hip.so")
-SOURCES = ["msp_kernels.hip", "msp_tridiag_lp.hip", "msp_tridiag_rs.hip", "msp_backtransform_mfma.hip", "msp_lindblad.hip",
+SOURCES = ["msp_kernels.hip", "msp_tridiag_lp.hip", "msp_tridiag_rs.hip", "msp_backtransform_mfma.hip", "msp_sbr.hip", "msp_lindblad.hip",
            "msp_mixed.hip", "msp_timesum_mfma.hip",
            "msp_adjoint.hip", "msp_gemm_mfma.hip", "msp_celio.hip", "msp_capi.hip"]
 # per-source code generation options (none at present)

@@ -28,22 +28,23 @@ namespace {
 constexpr int CW = 16;           // columns per workgroup
 static_assert(MSP_BT_NB == 16, "the MFMA back-transformation tiles reflector blocks of 16");
 
-// Reflector j of block k0 at row r (unit at r = k0 + j + 1, stored tail
+// Reflector j of block k0 at row r (unit at r = k0 + j + off, stored tail
 // below, zero above), read from a clamped address so that no branch
 // separates the loads of a k-step.
-__device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j)
+__device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j, int off)
 {
     const int k = k0 + j;
-    const bool tail = (r > k + 1) && (r < n) && (j < nbk);
+    const bool tail = (r > k + off) && (r < n) && (j < nbk);
     const int rc = tail ? r : n - 1, kc = (k < n) ? k : n - 1;
     const cdbl a = A[(size_t)rc * n + kc];
     if (tail) return a;
-    return cmk((r == k + 1 && j < nbk) ? 1.0 : 0.0, 0.0);
+    return cmk((r == k + off && j < nbk) ? 1.0 : 0.0, 0.0);
 }
 
 template <int RT, int NW>
 __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backtransform_reg(int n, const double *H, double *work,
-                                                               double *U)
+                                                               double *U, int off, int nref,
+                                                               int zc)
 {
     __shared__ double Wp[2][NW][2][256];      // per-wave partial W (re, im), by block parity
     __shared__ double Ws[(NW > 4) ? 512 : 1];  // NW > 4: W summed over the waves (re, im)
@@ -59,6 +60,7 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
     const double *Z = dc_result_ptr(n, base);
     const cdbl *VTall = dc_vt(n, base);
 
+    const cdbl *Zcb = zc ? dc_zc_ptr(n, base) : nullptr;
     msp_d4 ur[RT], ui[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
@@ -66,25 +68,31 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 16 * t + 4 * r + lq, col = c0 + li;
-            ur[i][r] = (t < ntiles && row < n && col < n) ? Z[(size_t)col * n + row] : 0.0;
-            ui[i][r] = 0.0;
+            const bool in = t < ntiles && row < n && col < n;
+            if (Zcb) {
+                const cdbl z = in ? Zcb[(size_t)col * n + row] : cmk(0.0, 0.0);
+                ur[i][r] = z.re;
+                ui[i][r] = z.im;
+            } else {
+                ur[i][r] = in ? Z[(size_t)col * n + row] : 0.0;
+                ui[i][r] = 0.0;
+            }
         }
     }
 
-    const int nref = n - 1;
     const int nblk = (nref + MSP_BT_NB - 1) / MSP_BT_NB;
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int k0 = blk * MSP_BT_NB;
         const int nbk = (nref - k0 < MSP_BT_NB) ? nref - k0 : MSP_BT_NB;
-        // ---- W = V^H U over this wave's tiles (rows >= k0 + 1 only matter)
+        // ---- W = V^H U over this wave's tiles (rows >= k0 + off only matter)
         msp_d4 wr = d4zero(), wi = d4zero();
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
             const int t = wave + NW * i;
-            if (t < ntiles && 16 * t + 15 >= k0 + 1) {
+            if (t < ntiles && 16 * t + 15 >= k0 + off) {
                 cdbl v[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + 4 * s + lq, li);
+                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + 4 * s + lq, li, off);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
                     mfma_cmac(v[s].re, -v[s].im, ur[i][s], ui[i][s], wr, wi);   // conj(V)
@@ -138,7 +146,7 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
             const int t = wave + NW * i;
-            if (t < ntiles && 16 * t + 15 >= k0 + 1) {
+            if (t < ntiles && 16 * t + 15 >= k0 + off) {
                 const int r = 16 * t + li;
                 const int rc = (r < n) ? r : n - 1;
                 cdbl v[4];
@@ -168,10 +176,11 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
 }
 
 template <int RT, int NW>
-hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U, hipStream_t s)
+hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U, hipStream_t s,
+                     int off, int nref, int zc)
 {
     hipLaunchKernelGGL((k_backtransform_reg<RT, NW>), dim3((n + CW - 1) / CW, batch), dim3(64 * NW),
-                       0, s, n, H, work, U);
+                       0, s, n, H, work, U, off, nref, zc);
     return hipGetLastError();
 }
 
@@ -179,14 +188,15 @@ hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U,
 
 // n <= 1024 (16-row tiles, at most 8 per wave; 8 waves above n = 512)
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
-                                     hipStream_t s)
+                                     hipStream_t s, int off, int nref, int zc)
 {
     if (n > 1024) return hipErrorNotSupported;
-    if (n > 512) return bt_launch<8, 8>(n, batch, H, work, U, s);
+    if (nref < 0) nref = n - off;
+    if (n > 512) return bt_launch<8, 8>(n, batch, H, work, U, s, off, nref, zc);
     const int per_wave = ((n + 15) / 16 + 3) / 4;
-    if (per_wave <= 1) return bt_launch<1, 4>(n, batch, H, work, U, s);
-    if (per_wave <= 2) return bt_launch<2, 4>(n, batch, H, work, U, s);
-    if (per_wave <= 3) return bt_launch<3, 4>(n, batch, H, work, U, s);
-    if (per_wave <= 4) return bt_launch<4, 4>(n, batch, H, work, U, s);
-    return bt_launch<8, 4>(n, batch, H, work, U, s);
+    if (per_wave <= 1) return bt_launch<1, 4>(n, batch, H, work, U, s, off, nref, zc);
+    if (per_wave <= 2) return bt_launch<2, 4>(n, batch, H, work, U, s, off, nref, zc);
+    if (per_wave <= 3) return bt_launch<3, 4>(n, batch, H, work, U, s, off, nref, zc);
+    if (per_wave <= 4) return bt_launch<4, 4>(n, batch, H, work, U, s, off, nref, zc);
+    return bt_launch<8, 4>(n, batch, H, work, U, s, off, nref, zc);
 }

@@ -184,6 +184,7 @@ size_t msp_eigh_workspace(int n, int batch, int method)
     if (n < 1 || batch < 1) return 0;
     if (method == MSP_EIGH_AUTO) method = (n <= kJacobiMax) ? MSP_EIGH_JACOBI : MSP_EIGH_DC;
     if (method == MSP_EIGH_JACOBI) return 0;
+    if (method == MSP_EIGH_DC2) return dc2_workspace_doubles(n) * sizeof(double) * (size_t)batch;
     return dc_workspace_doubles(n) * sizeof(double) * (size_t)batch;
 }
 
@@ -201,6 +202,17 @@ int msp_eigh(int n, int batch, double *d_H, double *d_E, double *d_U, int32_t *d
                         kJacobiMax, n);
         return hip_status(launch_eigh_jacobi(n, batch, d_H, d_E, d_U, d_info, 40, S(stream)),
                           "msp_eigh(jacobi)");
+    }
+    if (method == MSP_EIGH_DC2) {
+        if (n < kSbrMin || n > kSbrMax)
+            return fail(MSP_ERR_UNSUPPORTED, "msp_eigh: two-stage path holds %d <= n <= %d (n = %d)",
+                        kSbrMin, kSbrMax, n);
+        const size_t need2 = msp_eigh_workspace(n, batch, MSP_EIGH_DC2);
+        if (!d_work || work_bytes < need2)
+            return fail(MSP_ERR_VALUE, "msp_eigh: workspace of %zu bytes needed, %zu given", need2,
+                        work_bytes);
+        return hip_status(launch_eigh_dc2(n, batch, d_H, d_E, d_U, d_info, (double *)d_work, S(stream)),
+                          "msp_eigh(dc2)");
     }
     if (method != MSP_EIGH_DC) return fail(MSP_ERR_VALUE, "msp_eigh: unknown method %d", method);
     size_t need = msp_eigh_workspace(n, batch, MSP_EIGH_DC);

@@ -90,8 +90,30 @@ hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *wor
 // matrix) for 2 <= n <= 64, same outputs as launch_tridiag_lp.
 size_t tridiag_rs_lds_bytes(int n);
 hipError_t launch_tridiag_rs(int n, int batch, double *H, double *E, double *work, hipStream_t s);
+// U = Q Z on MFMA.  off: rows between a reflector's column and its unit
+// element (1 for the one-stage reduction, 16 for the band stage's panels);
+// nref reflectors; zc: start from the complex Z' = Q2 Z of the two-stage
+// reduction (dc_zc_ptr) instead of the divide and conquer's real Z.
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
-                                     hipStream_t s);
+                                     hipStream_t s, int off = 1, int nref = -1, int zc = 0);
+
+// Two-stage reduction (msp_sbr.hip) for kSbrMin <= n <= kSbrMax: dense ->
+// band of 16 (panel reflectors, tau and the V T blocks into the workspace as
+// the one-stage reduction leaves them, unit element 16 rows down), then band ->
+// real tridiagonal (d to E, e to the workspace, the chase reflectors to
+// `chase`: chase_doubles(n) per instance).
+constexpr int kSbrMin = 34;
+constexpr int kSbrMax = 256;
+hipError_t launch_sbr(int n, int batch, double *H, double *E, double *work, double *chase,
+                      hipStream_t s);
+// Z' = Q2 Z (the chase reflectors in reverse order) into each instance's
+// dc_zc_ptr (complex, column-major).
+hipError_t launch_chase_apply(int n, int batch, double *work, const double *chase, hipStream_t s);
+// msp_eigh with the two-stage reduction: workspace doubles per instance and
+// the whole pipeline (band, chase, divide and conquer, Q2, Q1).
+size_t dc2_workspace_doubles(int n);
+hipError_t launch_eigh_dc2(int n, int batch, double *H, double *E, double *U, int32_t *info,
+                           double *work, hipStream_t s);
 
 // Lindbladian path (msp_lindblad.hip)
 size_t lindblad_modes_workspace(int N, int batch);

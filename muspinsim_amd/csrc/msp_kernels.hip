@@ -1038,6 +1038,28 @@ static hipStream_t side_stream()
     return streams[dev];
 }
 
+// The divide and conquer on d (E) and e (workspace), eigenvectors into the
+// workspace (dc_result_ptr).
+static hipError_t launch_stedc(int n, int batch, double *E, double *work, int32_t *info,
+                               hipStream_t s)
+{
+    const int in_lds = (n <= 512) ? 1 : 0;
+    const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
+    // one 1024-thread workgroup per instance where the instance's LDS
+    // arrays leave room for one workgroup per CU (n >= 466) or live in
+    // global memory (n > 512): 256 threads would leave one wave per SIMD
+    // (n = 512: 9.8 -> 5.2 ms, n = 1024: 64 -> 28 ms per 140 instances)
+    const bool big = !in_lds || 2 * lds2 > 160 * 1024;
+    hipError_t err = msp_set_lds(big ? (const void *)k_stedc<1024> : (const void *)k_stedc<256>, lds2);
+    if (err != hipSuccess) return err;
+    ProfScope ps_("k_stedc", s);
+    if (big)
+        hipLaunchKernelGGL(k_stedc<1024>, dim3(batch), dim3(1024), lds2, s, n, E, work, info, in_lds);
+    else
+        hipLaunchKernelGGL(k_stedc<256>, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
+    return hipGetLastError();
+}
+
 // The tridiagonal eigensolver on s and, when s2 is given, the WY factors on
 // s2 (forked from s after the tridiagonalisation, joined back before the
 // back-transformation); every exit after the fork joins s2 into s.
@@ -1054,24 +1076,7 @@ static hipError_t eigh_dc_middle(int n, int batch, double *H, double *E, int32_t
         ProfScope ps_("k_larft", s2);
         err = launch_larft(n, batch, (const double *)H, work, s2);
     }
-    if (err == hipSuccess) {
-        const int in_lds = (n <= 512) ? 1 : 0;
-        const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
-        // one 1024-thread workgroup per instance where the instance's LDS
-        // arrays leave room for one workgroup per CU (n >= 466) or live in
-        // global memory (n > 512): 256 threads would leave one wave per SIMD
-        // (n = 512: 9.8 -> 5.2 ms, n = 1024: 64 -> 28 ms per 140 instances)
-        const bool big = !in_lds || 2 * lds2 > 160 * 1024;
-        err = msp_set_lds(big ? (const void *)k_stedc<1024> : (const void *)k_stedc<256>, lds2);
-        if (err == hipSuccess) {
-            ProfScope ps_("k_stedc", s);
-            if (big)
-                hipLaunchKernelGGL(k_stedc<1024>, dim3(batch), dim3(1024), lds2, s, n, E, work, info, in_lds);
-            else
-                hipLaunchKernelGGL(k_stedc<256>, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
-            err = hipGetLastError();
-        }
-    }
+    if (err == hipSuccess) err = launch_stedc(n, batch, E, work, info, s);
     if (forked) {
         // join even after a failure, so that no capture ends with s2 unjoined
         hipError_t e1 = hipEventRecord(ev_ft, s2);
@@ -1103,6 +1108,25 @@ hipError_t launch_eigh_dc(int n, int batch, double *H, double *E, double *U, int
     if (err != hipSuccess) return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
     return launch_eigh_from_tridiag(n, batch, H, E, U, info, work, s, overlap);
+}
+
+size_t dc2_workspace_doubles(int n) { return dc_work_doubles(n) + chase_doubles(n); }
+
+// Two-stage path: band and chase (d, e, the chase reflectors after the
+// per-instance arrays), divide and conquer, Z' = Q2 Z, then U = Q1 Z' with the
+// panel reflectors (unit element 16 rows below their column; the band stage
+// has already written their V T blocks, so no WY-factor kernel runs).
+hipError_t launch_eigh_dc2(int n, int batch, double *H, double *E, double *U, int32_t *info,
+                           double *work, hipStream_t s)
+{
+    if (n < kSbrMin || n > kSbrMax) return hipErrorNotSupported;
+    double *chase = work + (size_t)batch * dc_work_doubles(n);
+    hipError_t err = launch_sbr(n, batch, H, E, work, chase, s);
+    if (err != hipSuccess) return err;
+    if ((err = launch_stedc(n, batch, E, work, info, s)) != hipSuccess) return err;
+    if ((err = launch_chase_apply(n, batch, work, chase, s)) != hipSuccess) return err;
+    ProfScope ps_("k_backtransform", s);
+    return launch_backtransform_mfma(n, batch, (const double *)H, work, U, s, 16, n - 17, 1);
 }
 
 hipError_t launch_eigh_from_tridiag(int n, int batch, const double *R, double *E, double *U,

@@ -960,6 +960,15 @@ MSP_HD size_t dc_work_doubles(int n)
            2 * (size_t)bt_nblocks(n) * MSP_BT_NB * n;
 }
 
+// Chase reflectors of the two-stage reduction: sweep j (0 <= j <= n - 2) has
+// steps s = 0 and s >= 1 while j + 1 + 16 s <= n - 2; each stores 16 complex
+// entries of v and tau at index j * chase_steps(n) + s.
+MSP_HD int chase_steps(int n) { return 1 + (n >= 3 ? (n - 3) / 16 : 0); }
+MSP_HD size_t chase_doubles(int n)
+{
+    return (size_t)2 * (n > 1 ? n - 1 : 1) * chase_steps(n) * 17;
+}
+
 // Carve a DcWork (and e, tau) out of one contiguous per-matrix block.
 MSP_HD DcWork dc_carve(int n, double *base, double **e, cdbl **tau)
 {
@@ -1005,6 +1014,9 @@ MSP_HD cdbl *dc_vt(int n, double *base)
 // Direct offsets into the per-matrix workspace (same layout as dc_carve), for
 // kernels that need only a few of the arrays.
 MSP_HD cdbl *dc_tau_ptr(int n, double *base) { return (cdbl *)(base + n); }
+// Complex n x n (column-major) over the S and QK arrays, free once the divide
+// and conquer has finished: Z' = Q2 Z of the two-stage reduction.
+MSP_HD cdbl *dc_zc_ptr(int n, double *base) { return (cdbl *)(base + 3 * (size_t)n + 2 * (size_t)n * n); }
 MSP_HD double *dc_result_ptr(int n, double *base)
 {
     return base + 3 * (size_t)n + ((dc_depth(n) & 1) ? (size_t)n * n : 0);

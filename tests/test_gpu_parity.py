@@ -34,7 +34,9 @@ def _dev(a, dt=torch.complex128):
 
 @pytest.mark.parametrize("n,method", [(1, 1), (2, 1), (3, 1), (8, 1), (16, 1), (33, 1), (64, 1),
                                       (2, 2), (5, 2), (16, 2), (65, 2), (128, 2), (192, 2),
-                                      (256, 2), (300, 2), (600, 2), (1000, 2), (1100, 2)])
+                                      (256, 2), (300, 2), (600, 2), (1000, 2), (1100, 2),
+                                      (34, 3), (47, 3), (100, 3), (129, 3), (192, 3), (255, 3),
+                                      (256, 3)])
 def test_eigh_random(eng, n, method):
     rng = np.random.default_rng(n)
     Hs = np.array([_herm(rng, n) for _ in range(3)])
@@ -64,6 +66,33 @@ def test_eigh_degenerate_and_zero(eng):
         assert np.abs(H @ U[0] - U[0] * E[0]).max() < 1e-12
         assert np.abs(U[1].conj().T @ U[1] - np.eye(16)).max() < 1e-14
         assert np.all(E[1] == 0)
+
+
+def test_eigh_two_stage_structured(eng):
+    """Two-stage reduction (method 3) on matrices whose reflectors have zero
+    tails (tau = 0, length-one reflectors): a degenerate spin Hamiltonian, the
+    zero matrix, a diagonal and a tridiagonal matrix; and its size limits."""
+    from muspinsim_amd import _native as N
+    Is = [0.5] * 6
+    H = physics.hamiltonian(Is, [("bilinear", 0, k, np.eye(3) * 50.0 * k) for k in range(1, 6)])
+    rng = np.random.default_rng(5)
+    D = np.diag(rng.normal(size=64)).astype(complex)
+    T = np.diag(rng.normal(size=64)) + np.diag(rng.normal(size=63) + 1j * rng.normal(size=63), -1)
+    T = T + np.tril(T, -1).conj().T
+    mats = np.array([H, np.zeros_like(H), D, T])
+    E, U, info = eng.eigh(_dev(mats), method=3)
+    E, U = E.cpu().numpy(), U.cpu().numpy()
+    assert np.all(info.cpu().numpy() == 0)
+    for k, M in enumerate(mats):
+        Er = np.linalg.eigvalsh(M)
+        nrm = max(1.0, np.abs(Er).max())
+        assert np.abs(E[k] - Er).max() <= 1e-12 * nrm
+        assert np.abs(M @ U[k] - U[k] * E[k]).max() <= 1e-12 * nrm
+        assert np.abs(U[k].conj().T @ U[k] - np.eye(64)).max() <= 1e-13
+    assert np.all(E[1] == 0)
+    for n in (33, 257):
+        with pytest.raises(NotImplementedError, match="two-stage"):
+            eng.eigh(_dev(np.zeros((1, n, n))), method=3)
 
 
 def _build(eng, systems, batch, idx):
