@@ -41,6 +41,102 @@ __device__ inline cdbl vload(int n, const cdbl *A, int k0, int nbk, int r, int j
     return cmk((r == k + off && j < nbk) ? 1.0 : 0.0, 0.0);
 }
 
+// WY factors of reflector block blk (reflectors k0 = 16 blk .. k0 + 15) on
+// FP64 MFMA, operands read from global memory (8 KB of LDS, so that the
+// kernel shares CUs with the divide and conquer it runs beside):
+//   G = V^H V: each wave sums the tiles t = wave, wave + 4, ... of rows
+//       (B operand V(16 t + 4 kk + q, l), A operand its conjugate); the wave
+//       partials are added into LDS one wave after the other (fixed order);
+//   T = (diag(1/tau) + striu(G))^{-1} scaled by tau row-wise, one column per
+//       thread by back substitution (as the one-stage WY kernel);
+//   V T per tile on MFMA, written column by column for the back-transformation.
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_larft_mfma(int n, const double *H, double *work)
+{
+    __shared__ cdbl G[MSP_BT_NB * MSP_BT_NB];
+    __shared__ cdbl T[MSP_BT_NB * MSP_BT_NB];
+    int blk, b;
+    msp_xcd_tile(gridDim.x, gridDim.y, blk, b);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lq = lane >> 4;
+    const cdbl *A = (const cdbl *)H + (size_t)b * n * n;
+    double *base = work + (size_t)b * dc_work_doubles(n);
+    const cdbl *tau = dc_tau_ptr(n, base);
+    const int k0 = blk * MSP_BT_NB;
+    const int nref = n - 1;
+    const int nbk = (nref - k0 < MSP_BT_NB) ? nref - k0 : MSP_BT_NB;
+    const int ntiles = (n + 15) / 16;
+    const int t0 = (k0 + 1) / 16;          // first tile with rows >= k0 + 1
+    // ---- Gram partial of this wave
+    msp_d4 gr = d4zero(), gi = d4zero();
+    for (int t = t0 + wave; t < ntiles; t += NW) {
+        cdbl v[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) v[kk] = vload(n, A, k0, nbk, 16 * t + 4 * kk + lq, li, 1);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) mfma_cmac(v[kk].re, -v[kk].im, v[kk].re, v[kk].im, gr, gi);
+    }
+    for (int w2 = 0; w2 < NW; ++w2) {
+        if (wave == w2) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int o = (4 * r + lq) * 16 + li;
+                cdbl g = cmk(gr[r], gi[r]);
+                if (w2 > 0) g = cadd(g, G[o]);
+                G[o] = g;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- T by back substitution (lanes 0..15 of wave 0)
+    if (tid < MSP_BT_NB) {
+        const int j = tid;
+        cdbl tc[MSP_BT_NB];
+#pragma unroll
+        for (int i = MSP_BT_NB - 1; i >= 0; --i) {
+            cdbl t = cmk(0.0, 0.0);
+            if (j < nbk && i <= j) {
+                if (i == j) {
+                    t = tau[k0 + j];
+                } else {
+                    cdbl acc = cmk(0.0, 0.0);
+#pragma unroll
+                    for (int k = i + 1; k < MSP_BT_NB; ++k) cfma(acc, G[i * MSP_BT_NB + k], tc[k]);
+                    const cdbl ti = tau[k0 + i];
+                    t = cmul(cmk(-ti.re, -ti.im), acc);
+                }
+            }
+            tc[i] = t;
+        }
+#pragma unroll
+        for (int i = 0; i < MSP_BT_NB; ++i) T[i * MSP_BT_NB + j] = tc[i];
+    }
+    __syncthreads();
+    cdbl *Tout = dc_tblocks(n, base) + (size_t)blk * MSP_BT_NB * MSP_BT_NB;
+    if (tid < MSP_BT_NB * MSP_BT_NB) Tout[tid] = T[tid];
+    // ---- V T per tile: D = V_t T (A operand V(16 t + l, 4 kk + q), B operand
+    // T(4 kk + q, l)); rows above k0 + 1 are zero in V and are written as zeros
+    cdbl *VT = dc_vt(n, base) + (size_t)blk * MSP_BT_NB * n;
+    for (int t = wave; t < ntiles; t += NW) {
+        msp_d4 dr = d4zero(), di = d4zero();
+        if (t >= t0) {
+            cdbl v[4], tt[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                v[kk] = vload(n, A, k0, nbk, 16 * t + li, 4 * kk + lq, 1);
+                tt[kk] = T[(4 * kk + lq) * MSP_BT_NB + li];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) mfma_cmac(v[kk].re, v[kk].im, tt[kk].re, tt[kk].im, dr, di);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * t + 4 * r + lq;
+            if (row < n) VT[(size_t)li * n + row] = cmk(dr[r], di[r]);
+        }
+    }
+}
+
 template <int RT, int NW>
 __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backtransform_reg(int n, const double *H, double *work,
                                                                double *U, int off, int nref,
@@ -185,6 +281,13 @@ hipError_t bt_launch(int n, int batch, const double *H, double *work, double *U,
 }
 
 }  // namespace
+
+// WY factors and V T of every reflector block of the one-stage reduction
+hipError_t launch_larft_mfma(int n, int batch, const double *H, double *work, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_larft_mfma<4>, dim3(bt_nblocks(n), batch), dim3(256), 0, s, n, H, work);
+    return hipGetLastError();
+}
 
 // n <= 1024 (16-row tiles, at most 8 per wave; 8 waves above n = 512)
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,

@@ -97,6 +97,10 @@ hipError_t launch_tridiag_rs(int n, int batch, double *H, double *E, double *wor
 hipError_t launch_backtransform_mfma(int n, int batch, const double *H, double *work, double *U,
                                      hipStream_t s, int off = 1, int nref = -1, int zc = 0);
 
+// WY factors (T blocks) and V T of the one-stage reflectors on MFMA, reading
+// the reflectors from H directly (no LDS staging).
+hipError_t launch_larft_mfma(int n, int batch, const double *H, double *work, hipStream_t s);
+
 // Two-stage reduction (msp_sbr.hip) for kSbrMin <= n <= kSbrMax: dense ->
 // band of 16 (panel reflectors, tau and the V T blocks into the workspace as
 // the one-stage reduction leaves them, unit element 16 rows down), then band ->

@@ -494,6 +494,7 @@ constexpr int kLarftStageMax = kLarftStageMaxDev;
 
 hipError_t launch_larft(int n, int batch, const double *H, double *work, hipStream_t s)
 {
+    if (n <= 1024) return launch_larft_mfma(n, batch, H, work, s);
     if (n <= kLarftStageMax) {
         const size_t lds = sizeof(cdbl) * (size_t)n * (MSP_BT_NB + 1);
         hipError_t err = msp_set_lds((const void *)k_larft<true>, lds);
