@@ -135,7 +135,7 @@ int hc_eig(int n, const double *L, double *lam, double *V, const double *r, cons
     memcpy(A.data(), L, sizeof(cdbl) * n * n);
     double red[1];
     hessenberg_team(n, A.data(), n, Z.data(), n, v.data(), w.data(), red);
-    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its);
+    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its, red);
     if (info) return info;
     schur_vectors_team(n, A.data(), n, (const cdbl *)lam);
     modes_coeff_team(n, A.data(), n, Z.data(), n, (const cdbl *)r, y.data());
@@ -146,6 +146,27 @@ int hc_eig(int n, const double *L, double *lam, double *V, const double *r, cons
     return 0;
 }
 
+// The mode amplitudes without Z: r and u carried through the reduction as
+// Z^H r and u^T Z (the device's LDS form when no eigenvectors are wanted).
+int hc_eig_vec(int n, const double *L, double *lam, const double *r, const double *u, double *c,
+               double *o, int max_its)
+{
+    std::vector<cdbl> A((size_t)n * n), v(n), w(n), y(n), t(n), rho(n), om(n);
+    memcpy(A.data(), L, sizeof(cdbl) * n * n);
+    memcpy(rho.data(), r, sizeof(cdbl) * n);
+    memcpy(om.data(), u, sizeof(cdbl) * n);
+    double red[1];
+    ModeVecs mv{rho.data(), om.data(), 1};
+    hessenberg_team(n, A.data(), n, nullptr, 0, v.data(), w.data(), red, mv);
+    int info = schur_team(n, A.data(), n, nullptr, 0, (cdbl *)lam, max_its, red, mv);
+    if (info) return info;
+    schur_vectors_team(n, A.data(), n, (const cdbl *)lam);
+    modes_coeff_team(n, A.data(), n, nullptr, 0, rho.data(), y.data());
+    memcpy(c, y.data(), sizeof(cdbl) * n);
+    modes_proj_team(n, A.data(), n, nullptr, 0, om.data(), t.data(), (cdbl *)o);
+    return 0;
+}
+
 // Hessenberg + Schur only: T (upper triangular, lower part zeroed) and Z.
 int hc_schur(int n, const double *L, double *T, double *Zout, double *lam, int max_its)
 {
@@ -153,7 +174,7 @@ int hc_schur(int n, const double *L, double *T, double *Zout, double *lam, int m
     memcpy(A.data(), L, sizeof(cdbl) * n * n);
     double red[1];
     hessenberg_team(n, A.data(), n, Z.data(), n, v.data(), w.data(), red);
-    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its);
+    int info = schur_team(n, A.data(), n, Z.data(), n, (cdbl *)lam, max_its, red);
     memcpy(T, A.data(), sizeof(cdbl) * n * n);
     memcpy(Zout, Z.data(), sizeof(cdbl) * n * n);
     return info;

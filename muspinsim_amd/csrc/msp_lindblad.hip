@@ -72,22 +72,29 @@ __global__ void __launch_bounds__(kNT) k_liouvillian(int n, const double *H, int
 // plus Z (N x N) when the matrix is not in LDS.
 __host__ __device__ inline size_t modes_vec_elems(int N) { return 6 * (size_t)N; }
 
-__global__ void __launch_bounds__(kNT) k_lindblad_modes(int N, double *Lg, const double *r,
-                                                        int n_ops, const double *u, double *lam,
-                                                        double *amp, double *cout, double *Vout,
-                                                        int32_t *info, double *work, int in_lds,
-                                                        int max_its)
+// NT threads per instance: one wave when the matrix is in LDS (N <= 64: a
+// sweep step's rows and columns are one element per lane, and the barriers of
+// the QR steps are those of a single wave), four above.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_lindblad_modes(int N, double *Lg, const double *r,
+                                                       int n_ops, const double *u, double *lam,
+                                                       double *amp, double *cout, double *Vout,
+                                                       int32_t *info, double *work, int in_lds,
+                                                       int max_its)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double red[kNT];
+    __shared__ double red[NT];
     const int b = blockIdx.x;
     cdbl *A, *Z, *vec;
     int ld;
+    // without eigenvectors to return, an LDS-resident matrix carries Z^H r and
+    // u_q^T Z through the reduction instead of Z (half the LDS)
+    const bool vmode = in_lds && !Vout;
     if (in_lds) {
         ld = N + 1;
         A = (cdbl *)smem;
-        Z = A + (size_t)N * ld;
-        vec = Z + (size_t)N * ld;
+        Z = vmode ? nullptr : A + (size_t)N * ld;
+        vec = vmode ? A + (size_t)N * ld : Z + (size_t)N * ld;
         const cdbl *Lb = (const cdbl *)Lg + (size_t)b * N * N;
         MSP_PFOR(idx, N * N)
         {
@@ -103,19 +110,32 @@ __global__ void __launch_bounds__(kNT) k_lindblad_modes(int N, double *Lg, const
     }
     cdbl *v = vec, *w = vec + N, *y = vec + 2 * N, *t = vec + 3 * N, *rv = vec + 4 * N,
          *lv = vec + 5 * N;
+    ModeVecs mv{nullptr, nullptr, 0};
+    if (vmode) {
+        mv.rho = rv;
+        mv.omega = vec + 6 * N;
+        mv.n_omega = n_ops;
+        MSP_PFOR(i, N) rv[i] = ((const cdbl *)r)[(size_t)b * N + i];
+        MSP_PFOR(i, n_ops * N) mv.omega[i] = ((const cdbl *)u)[(size_t)b * n_ops * N + i];
+    }
     MSP_SYNC();
-    hessenberg_team(N, A, ld, Z, ld, v, w, red);
-    const int st = schur_team(N, A, ld, Z, ld, lv, max_its);
+    hessenberg_team(N, A, ld, Z, ld, v, w, red, mv);
+    const int st = schur_team(N, A, ld, Z, ld, lv, max_its, red, mv);
     if (threadIdx.x == 0) info[b] = st;
     if (st != 0) return;                      // uniform across the workgroup
     schur_vectors_team(N, A, ld, lv);
-    MSP_PFOR(i, N) rv[i] = ((const cdbl *)r)[(size_t)b * N + i];
-    MSP_SYNC();
+    if (!vmode) {
+        MSP_PFOR(i, N) rv[i] = ((const cdbl *)r)[(size_t)b * N + i];
+        MSP_SYNC();
+    }
     modes_coeff_team(N, A, ld, Z, ld, rv, y);          // y = c
     for (int k = 0; k < n_ops; ++k) {
-        MSP_PFOR(i, N) rv[i] = ((const cdbl *)u)[((size_t)b * n_ops + k) * N + i];
-        MSP_SYNC();
-        modes_proj_team(N, A, ld, Z, ld, rv, t, w);    // w = o
+        const cdbl *uk = vmode ? mv.omega + (size_t)k * N : rv;
+        if (!vmode) {
+            MSP_PFOR(i, N) rv[i] = ((const cdbl *)u)[((size_t)b * n_ops + k) * N + i];
+            MSP_SYNC();
+        }
+        modes_proj_team(N, A, ld, Z, ld, uk, t, w);    // w = o
         MSP_PFOR(m, N) ((cdbl *)amp)[((size_t)b * n_ops + k) * N + m] = cmul(w[m], y[m]);
         MSP_SYNC();
     }
@@ -221,12 +241,21 @@ hipError_t launch_lindblad_modes(int N, int batch, double *L, const double *r, i
                                  int32_t *info, double *work, int max_its, hipStream_t s)
 {
     const int in_lds = N <= kLdsMaxN;
-    const size_t lds = in_lds ? sizeof(cdbl) * (2 * (size_t)N * (N + 1) + modes_vec_elems(N)) : 0;
-    hipError_t err = msp_set_lds((const void *)k_lindblad_modes, lds);
+    // (without V the LDS form holds the matrix, the vectors and the n_ops rows
+    // u_q^T Z, not Z)
+    const size_t lds = !in_lds ? 0
+                       : V ? sizeof(cdbl) * (2 * (size_t)N * (N + 1) + modes_vec_elems(N))
+                           : sizeof(cdbl) * ((size_t)N * (N + 1) + modes_vec_elems(N) + (size_t)n_ops * N);
+    const void *fn = in_lds ? (const void *)k_lindblad_modes<64> : (const void *)k_lindblad_modes<kNT>;
+    hipError_t err = msp_set_lds(fn, lds);
     if (err != hipSuccess) return err;
     ProfScope ps_("k_lindblad_modes", s);
-    hipLaunchKernelGGL(k_lindblad_modes, dim3(batch), dim3(kNT), lds, s, N, L, r, n_ops, u, lam,
-                       amp, c, V, info, work, in_lds, max_its);
+    if (in_lds)
+        hipLaunchKernelGGL(k_lindblad_modes<64>, dim3(batch), dim3(64), lds, s, N, L, r, n_ops, u,
+                           lam, amp, c, V, info, work, in_lds, max_its);
+    else
+        hipLaunchKernelGGL(k_lindblad_modes<kNT>, dim3(batch), dim3(kNT), lds, s, N, L, r, n_ops, u,
+                           lam, amp, c, V, info, work, in_lds, max_its);
     return hipGetLastError();
 }
 

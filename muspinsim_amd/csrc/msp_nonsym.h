@@ -21,30 +21,122 @@
 
 #include "msp_team.h"
 
+// Phase boundary of the team functions below.  A team of one wave (the
+// LDS-resident form: every shared array is in LDS, whose operations a wave
+// executes in issue order) needs only the compiler not to move memory
+// operations across it; larger teams take a workgroup barrier.
+#ifdef MSP_HOST_BUILD
+#define NS_SYNC() ((void)0)
+#else
+#define NS_SYNC() ns_sync_dev()
+__device__ inline void ns_sync_dev()
+{
+    if (blockDim.x <= 64) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    else __syncthreads();
+}
+#endif
+
+// Team sum; a one-wave team reduces with DPP lane moves (no LDS).
+MSP_HD double ns_team_sum(double v, double *red)
+{
+#ifdef MSP_HOST_BUILD
+    return team_sum(v, red);
+#else
+    if (blockDim.x > 64) return team_sum(v, red);
+    return wave_sum_all(v);
+#endif
+}
+
+// Team maximum; a one-wave team reduces with DPP lane moves (no LDS).
+MSP_HD double ns_team_max(double v, double *red)
+{
+#ifdef MSP_HOST_BUILD
+    return team_max(v, red);
+#else
+    if (blockDim.x > 64) return team_max(v, red);
+    v = fmax(v, dpp_f64<0xB1, 0xF>(v));    // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4E, 0xF>(v));    // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x141, 0xF>(v));   // row_half_mirror
+    v = fmax(v, dpp_f64<0x140, 0xF>(v));   // row_mirror
+    v = fmax(v, dpp_f64<0x142, 0xA>(v));   // row_bcast15 -> rows 1, 3
+    v = fmax(v, dpp_f64<0x143, 0xC>(v));   // row_bcast31 -> rows 2, 3
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+#endif
+}
+
 // |re| + |im|, the LAPACK cabs1 magnitude used in deflation tests
 MSP_HD double cabs1(cdbl a) { return fabs(a.re) + fabs(a.im); }
 MSP_HD double cabs(cdbl a) { return hypot(a.re, a.im); }
 
 // ---------------------------------------------------------------------------
-// Hessenberg reduction: A <- Q^H A Q (upper Hessenberg), Z <- Q.
-// v, w: team scratch of n complex each; red: team scratch of MSP_NT doubles.
-MSP_HD void hessenberg_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *v, cdbl *w,
-                            double *red)
+// Vectors carried through the reduction instead of Z (when the eigenvectors
+// themselves are not wanted): rho = Z^H r and the rows omega_q = u_q^T Z,
+// updated with every reflector and rotation that would update Z.
+struct ModeVecs {
+    cdbl *rho;      // n (starts as r)
+    cdbl *omega;    // n_omega x n, row q at q n (start as u_q)
+    int n_omega;
+};
+
+// rho <- H^H rho and omega_q <- omega_q H for H = I - tau v v^H (v on rows
+// k+1 .. n-1): one thread per vector forms its dot, then every thread updates.
+MSP_HD void modevecs_reflect(int n, int k, const cdbl *v, cdbl tau, ModeVecs mv, cdbl *dots)
 {
-    MSP_PFOR(idx, n * n)
+    const int nv = 1 + mv.n_omega;
+    MSP_PFOR(q, nv)
     {
-        const int i = idx / n, j = idx - (idx / n) * n;
-        Z[i * ldz + j] = cmk(i == j ? 1.0 : 0.0, 0.0);
+        cdbl s = cmk(0.0, 0.0);
+        if (q == 0) {
+#pragma unroll 8
+            for (int r = k + 1; r < n; ++r) cfmac(s, v[r], mv.rho[r]);      // v^H rho
+            dots[0] = cmul(cconj(tau), s);
+        } else {
+            const cdbl *om = mv.omega + (size_t)(q - 1) * n;
+#pragma unroll 8
+            for (int r = k + 1; r < n; ++r) cfma(s, om[r], v[r]);           // omega v
+            dots[q] = cmul(s, tau);
+        }
     }
-    MSP_SYNC();
+    NS_SYNC();
+    MSP_PFOR(idx, nv * (n - k - 1))
+    {
+        const int q = idx / (n - k - 1), r = k + 1 + idx % (n - k - 1);
+        if (q == 0) mv.rho[r] = csub(mv.rho[r], cmul(v[r], dots[0]));
+        else {
+            cdbl *om = mv.omega + (size_t)(q - 1) * n;
+            om[r] = csub(om[r], cmul(dots[q], cconj(v[r])));
+        }
+    }
+    NS_SYNC();
+}
+
+// ---------------------------------------------------------------------------
+// Hessenberg reduction: A <- Q^H A Q (upper Hessenberg), and Z <- Q, or
+// (Z null) the mode vectors mv transformed with Q.
+// v, w: team scratch of n complex each; red: team scratch of MSP_NT doubles.
+// Every update loop gives a thread whole columns or rows (no index division),
+// with the inner dots unrolled so several LDS loads are in flight.
+MSP_HD void hessenberg_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *v, cdbl *w,
+                            double *red, ModeVecs mv = ModeVecs{nullptr, nullptr, 0})
+{
+    if (Z) {
+        MSP_PFOR(i, n)
+        {
+            for (int j = 0; j < n; ++j) Z[i * ldz + j] = cmk(i == j ? 1.0 : 0.0, 0.0);
+        }
+    }
+    NS_SYNC();
     for (int k = 0; k + 2 < n; ++k) {
         // reflector H = I - tau v v^H with H^H [alpha; x] = [beta; 0]
         double part = 0.0;
         MSP_PFOR(i, n - k - 2) part += cabs2(A[(k + 2 + i) * lda + k]);
-        const double xn2 = team_sum(part, red);
+        const double xn2 = ns_team_sum(part, red);
         const cdbl alpha = A[(k + 1) * lda + k];
         if (xn2 == 0.0 && alpha.im == 0.0) {
-            MSP_SYNC();
+            NS_SYNC();
             continue;                                  // already reduced: tau = 0
         }
         const double anorm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
@@ -56,107 +148,124 @@ MSP_HD void hessenberg_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *v, 
             const int r = k + 1 + i;
             v[r] = (i == 0) ? cmk(1.0, 0.0) : cmul(A[r * lda + k], scal);
         }
-        MSP_SYNC();
+        NS_SYNC();
         // left: A[k+1.., j] -= conj(tau) v (v^H A[k+1.., j]), columns j > k
         MSP_PFOR(jj, n - k - 1)
         {
             const int j = k + 1 + jj;
             cdbl s = cmk(0.0, 0.0);
+#pragma unroll 8
             for (int r = k + 1; r < n; ++r) cfmac(s, v[r], A[r * lda + j]);
             w[j] = cmul(cconj(tau), s);
         }
-        MSP_SYNC();
-        MSP_PFOR(idx, (n - k - 1) * (n - k))
+        NS_SYNC();
+        MSP_PFOR(jj, n - k)
         {
-            const int r = k + 1 + idx / (n - k), j = k + idx % (n - k);
-            if (j == k) A[r * lda + k] = (r == k + 1) ? cmk(beta, 0.0) : cmk(0.0, 0.0);
-            else A[r * lda + j] = csub(A[r * lda + j], cmul(v[r], w[j]));
+            const int j = k + jj;
+            if (j == k) {
+                A[(k + 1) * lda + k] = cmk(beta, 0.0);
+                for (int r = k + 2; r < n; ++r) A[r * lda + k] = cmk(0.0, 0.0);
+            } else {
+                const cdbl wj = w[j];
+#pragma unroll 8
+                for (int r = k + 1; r < n; ++r) A[r * lda + j] = csub(A[r * lda + j], cmul(v[r], wj));
+            }
         }
-        MSP_SYNC();
-        // right: A[i, k+1..] -= (A[i, k+1..] v) tau v^H for every row; then Z
+        NS_SYNC();
+        // right: A[i, k+1..] -= (A[i, k+1..] v) tau v^H for every row
         MSP_PFOR(i, n)
         {
             cdbl s = cmk(0.0, 0.0);
+#pragma unroll 8
             for (int r = k + 1; r < n; ++r) cfma(s, A[i * lda + r], v[r]);
-            w[i] = cmul(s, tau);
+            const cdbl wi = cmul(s, tau);
+#pragma unroll 8
+            for (int r = k + 1; r < n; ++r) A[i * lda + r] = csub(A[i * lda + r], cmul(wi, cconj(v[r])));
         }
-        MSP_SYNC();
-        MSP_PFOR(idx, n * (n - k - 1))
-        {
-            const int i = idx / (n - k - 1), r = k + 1 + idx % (n - k - 1);
-            A[i * lda + r] = csub(A[i * lda + r], cmul(w[i], cconj(v[r])));
+        NS_SYNC();
+        if (Z) {
+            MSP_PFOR(i, n)
+            {
+                cdbl s = cmk(0.0, 0.0);
+#pragma unroll 8
+                for (int r = k + 1; r < n; ++r) cfma(s, Z[i * ldz + r], v[r]);
+                const cdbl wi = cmul(s, tau);
+#pragma unroll 8
+                for (int r = k + 1; r < n; ++r) Z[i * ldz + r] = csub(Z[i * ldz + r], cmul(wi, cconj(v[r])));
+            }
+            NS_SYNC();
+        } else {
+            modevecs_reflect(n, k, v, tau, mv, w);
         }
-        MSP_SYNC();
-        MSP_PFOR(i, n)
-        {
-            cdbl s = cmk(0.0, 0.0);
-            for (int r = k + 1; r < n; ++r) cfma(s, Z[i * ldz + r], v[r]);
-            w[i] = cmul(s, tau);
-        }
-        MSP_SYNC();
-        MSP_PFOR(idx, n * (n - k - 1))
-        {
-            const int i = idx / (n - k - 1), r = k + 1 + idx % (n - k - 1);
-            Z[i * ldz + r] = csub(Z[i * ldz + r], cmul(w[i], cconj(v[r])));
-        }
-        MSP_SYNC();
     }
 }
 
-// Givens rotation G = [c s; -conj(s) c] with G [f; g] = [r; 0], c real.
+// Givens rotation G = [c s; -conj(s) c] with G [f; g] = [r; 0], c real:
+// c = |f| / nrm, s = f conj(g) / (|f| nrm), r = f nrm / |f| with
+// nrm = sqrt(|f|^2 + |g|^2), formed from f and g scaled by their largest
+// component (two divisions and two square roots in all).
 MSP_HD void givens(cdbl f, cdbl g, double &c, cdbl &s, cdbl &r)
 {
+    const double sc = fmax(fmax(fabs(f.re), fabs(f.im)), fmax(fabs(g.re), fabs(g.im)));
     if (g.re == 0.0 && g.im == 0.0) {
         c = 1.0;
         s = cmk(0.0, 0.0);
         r = f;
         return;
     }
+    const double isc = 1.0 / sc;
+    const cdbl fs = cscale(f, isc), gs = cscale(g, isc);
+    const double f2 = fs.re * fs.re + fs.im * fs.im, g2 = gs.re * gs.re + gs.im * gs.im;
     if (f.re == 0.0 && f.im == 0.0) {
-        const double ga = cabs(g);
+        const double ga = sqrt(g2);
+        const double ig = 1.0 / ga;
         c = 0.0;
-        s = cmk(g.re / ga, -g.im / ga);
-        r = cmk(ga, 0.0);
+        s = cmk(gs.re * ig, -gs.im * ig);
+        r = cmk(ga * sc, 0.0);
         return;
     }
-    const double fa = cabs(f), ga = cabs(g);
-    const double nrm = hypot(fa, ga);
-    const cdbl fs = cmk(f.re / fa, f.im / fa);       // f / |f|
-    c = fa / nrm;
-    s = cscale(cmul(fs, cconj(g)), 1.0 / nrm);
-    r = cscale(fs, nrm);
+    const double fa = sqrt(f2), nrm = sqrt(f2 + g2);
+    const double inv = 1.0 / (fa * nrm);
+    c = f2 * inv;                                      // |f| / nrm
+    s = cscale(cmul(fs, cconj(gs)), inv);
+    r = cscale(f, (f2 + g2) * inv);                    // f nrm / |f|
 }
 
 // ---------------------------------------------------------------------------
 // Upper Hessenberg A -> upper triangular T = Z^H A Z by single-shift QR with
 // Wilkinson shifts (exceptional shifts after 10 and 20 stalled sweeps).
 // lam receives the diagonal.  Returns 0, or the 1-based index of an
-// unconverged eigenvalue after max_its sweeps per eigenvalue.
-MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int max_its)
+// unconverged eigenvalue after max_its sweeps per eigenvalue.  With Z null
+// the rotations go to the mode vectors mv instead.  red: team scratch of
+// MSP_NT doubles (the split search is a team maximum over the subdiagonal).
+MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int max_its,
+                      double *red, ModeVecs mv = ModeVecs{nullptr, nullptr, 0})
 {
     const double ulp = 1.1102230246251565e-16;
     const double smlnum = 2.2250738585072014e-308 * ((double)n / ulp);
     int ihi = n - 1, its = 0, total = 0;
     while (ihi >= 0) {
-        // split point: largest l <= ihi with A[l][l-1] negligible
-        int l = ihi;
-        for (; l > 0; --l) {
+        // split point: largest l <= ihi with A[l][l-1] negligible (0 if none),
+        // every l tested at once
+        double best = 0.0;
+        MSP_PFOR(t, ihi)
+        {
+            const int l = t + 1;
             const double sub = cabs1(A[l * lda + l - 1]);
-            if (sub <= smlnum) break;
             double tst = cabs1(A[(l - 1) * lda + l - 1]) + cabs1(A[l * lda + l]);
             if (tst == 0.0) {
                 if (l >= 2) tst += fabs(A[(l - 1) * lda + l - 2].re);
                 if (l + 1 <= ihi) tst += fabs(A[(l + 1) * lda + l].re);
             }
-            if (sub <= ulp * tst) break;
+            if (sub <= smlnum || sub <= ulp * tst) best = fmax(best, (double)l);
         }
-        MSP_SYNC();
+        const int l = (int)ns_team_max(best, red);
         if (l > 0) {
             if (MSP_TID == 0) A[l * lda + l - 1] = cmk(0.0, 0.0);
         }
         if (l == ihi) {
             if (MSP_TID == 0) lam[ihi] = A[ihi * lda + ihi];
-            MSP_SYNC();
+            NS_SYNC();
             --ihi;
             its = 0;
             continue;
@@ -192,7 +301,7 @@ MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int 
         }
         ++its;
         // single-shift sweep over rows/cols l .. ihi, applied to the full
-        // matrix (columns up to n-1, rows from 0) and to Z
+        // matrix (columns up to n-1, rows from 0) and to Z or the mode vectors
         for (int k = l; k < ihi; ++k) {
             cdbl f, g;
             if (k == l) {
@@ -205,7 +314,7 @@ MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int 
             double c;
             cdbl s, r;
             givens(f, g, c, s, r);
-            MSP_SYNC();
+            NS_SYNC();
             // rows k, k+1: [a; b] <- [c a + s b; -conj(s) a + c b]
             const int j0 = (k == l) ? k : k - 1;
             MSP_PFOR(jj, n - j0)
@@ -220,24 +329,43 @@ MSP_HD int schur_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *lam, int 
                     A[(k + 1) * lda + j] = csub(cscale(b, c), cmul(cconj(s), a));
                 }
             }
-            MSP_SYNC();
+            NS_SYNC();
             // columns k, k+1 of A (rows 0 .. min(k+2, ihi)) and of Z (all rows):
-            // [p q] <- [c p + conj(s) q, -s p + c q]
+            // [p q] <- [c p + conj(s) q, -s p + c q]; without Z, rho takes the
+            // row rotation and each omega_q the column rotation
             const int iend = (k + 2 < ihi) ? k + 2 : ihi;
-            MSP_PFOR(i2, (iend + 1) + n)
+            MSP_PFOR(i, iend + 1)
             {
-                cdbl *M;
-                int ld, i;
-                if (i2 <= iend) { M = A; ld = lda; i = i2; }
-                else { M = Z; ld = ldz; i = i2 - iend - 1; }
-                const cdbl p = M[i * ld + k], q = M[i * ld + k + 1];
-                M[i * ld + k] = cadd(cscale(p, c), cmul(cconj(s), q));
-                M[i * ld + k + 1] = csub(cscale(q, c), cmul(s, p));
+                const cdbl p = A[i * lda + k], q = A[i * lda + k + 1];
+                A[i * lda + k] = cadd(cscale(p, c), cmul(cconj(s), q));
+                A[i * lda + k + 1] = csub(cscale(q, c), cmul(s, p));
             }
-            MSP_SYNC();
+            if (Z) {
+                MSP_PFOR(i, n)
+                {
+                    const cdbl p = Z[i * ldz + k], q = Z[i * ldz + k + 1];
+                    Z[i * ldz + k] = cadd(cscale(p, c), cmul(cconj(s), q));
+                    Z[i * ldz + k + 1] = csub(cscale(q, c), cmul(s, p));
+                }
+            } else {
+                MSP_PFOR(q2, 1 + mv.n_omega)
+                {
+                    if (q2 == 0) {
+                        const cdbl a = mv.rho[k], b = mv.rho[k + 1];
+                        mv.rho[k] = cadd(cscale(a, c), cmul(s, b));
+                        mv.rho[k + 1] = csub(cscale(b, c), cmul(cconj(s), a));
+                    } else {
+                        cdbl *om = mv.omega + (size_t)(q2 - 1) * n;
+                        const cdbl p = om[k], q = om[k + 1];
+                        om[k] = cadd(cscale(p, c), cmul(cconj(s), q));
+                        om[k + 1] = csub(cscale(q, c), cmul(s, p));
+                    }
+                }
+            }
+            NS_SYNC();
         }
     }
-    MSP_SYNC();
+    NS_SYNC();
     return 0;
 }
 
@@ -256,13 +384,14 @@ MSP_HD void schur_vectors_team(int n, cdbl *A, int lda, const cdbl *lam)
         // x_i for i = k-1 .. 0; x stored in the lower triangle row k
         for (int i = k - 1; i >= 0; --i) {
             cdbl s = A[i * lda + k];                  // T_ik * x_k (x_k = 1)
+#pragma unroll 8
             for (int j = i + 1; j < k; ++j) cfma(s, A[i * lda + j], A[k * lda + j]);
             cdbl den = csub(lam[i], lk);
             if (cabs1(den) < smin) den = cmk(smin, 0.0);
             A[k * lda + i] = cscale(cdivc(s, den), -1.0);
         }
     }
-    MSP_SYNC();
+    NS_SYNC();
 }
 
 // X[i][k] accessor (unit diagonal, zero below)
@@ -280,15 +409,20 @@ MSP_HD void modes_coeff_team(int n, const cdbl *A, int lda, const cdbl *Z, int l
     MSP_PFOR(m, n)
     {
         cdbl s = cmk(0.0, 0.0);
-        for (int i = 0; i < n; ++i) cfmac(s, Z[i * ldz + m], r[i]);
+        if (Z) {
+#pragma unroll 8
+            for (int i = 0; i < n; ++i) cfmac(s, Z[i * ldz + m], r[i]);
+        } else {
+            s = r[m];                       // r is already Z^H r
+        }
         y[m] = s;
     }
-    MSP_SYNC();
+    NS_SYNC();
     // back substitution with the unit upper triangular X, column oriented
     for (int j = n - 1; j > 0; --j) {
         const cdbl cj = y[j];
         MSP_PFOR(i, j) y[i] = csub(y[i], cmul(A[j * lda + i], cj));
-        MSP_SYNC();
+        NS_SYNC();
     }
 }
 
@@ -299,17 +433,23 @@ MSP_HD void modes_proj_team(int n, const cdbl *A, int lda, const cdbl *Z, int ld
     MSP_PFOR(m, n)
     {
         cdbl s = cmk(0.0, 0.0);
-        for (int i = 0; i < n; ++i) cfma(s, u[i], Z[i * ldz + m]);
+        if (Z) {
+#pragma unroll 8
+            for (int i = 0; i < n; ++i) cfma(s, u[i], Z[i * ldz + m]);
+        } else {
+            s = u[m];                       // u is already u^T Z
+        }
         t[m] = s;
     }
-    MSP_SYNC();
+    NS_SYNC();
     MSP_PFOR(k, n)
     {
         cdbl s = t[k];
+#pragma unroll 8
         for (int j = 0; j < k; ++j) cfma(s, t[j], A[k * lda + j]);
         o[k] = s;
     }
-    MSP_SYNC();
+    NS_SYNC();
 }
 
 // V = Z X in place in Z (row by row, columns in descending order).
@@ -323,5 +463,5 @@ MSP_HD void modes_vectors_team(int n, const cdbl *A, int lda, cdbl *Z, int ldz)
             Z[i * ldz + k] = s;
         }
     }
-    MSP_SYNC();
+    NS_SYNC();
 }

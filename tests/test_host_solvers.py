@@ -130,3 +130,40 @@ def test_physical_hamiltonians(hc):
     U = np.zeros((192, 192), complex)
     hc.hc_eigh_dc_blocked(192, _p(H), _p(E), _p(U), 16, 16)
     _check(H, E, U, 1e-14)
+
+
+def _modes_ref(L, r, u):
+    """lambda_m and amp_m = (u^T V)_m (V^{-1} r)_m from numpy.linalg.eig (the
+    reference's Lindbladian.evolve, lindbladian.py:84-93); amp does not depend
+    on the scaling of V's columns."""
+    lam, V = np.linalg.eig(L)
+    amp = (u @ V) * np.linalg.solve(V, r)
+    order = np.lexsort((lam.imag, lam.real))
+    return lam[order], amp[order]
+
+
+@pytest.mark.parametrize("n", [4, 9, 16, 36, 64])
+def test_mode_amplitudes_with_and_without_schur_vectors(hc, n):
+    """Non-Hermitian eigendecomposition: the amplitudes from the explicit
+    Schur vectors Z (hc_eig) and from r and u carried through the reduction
+    instead of Z (hc_eig_vec, the device's LDS form without eigenvectors)
+    both match numpy.linalg.eig + solve."""
+    rng = np.random.default_rng(70 + n)
+    L = np.ascontiguousarray(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    r = np.ascontiguousarray(rng.normal(size=n) + 1j * rng.normal(size=n))
+    u = np.ascontiguousarray(rng.normal(size=n) + 1j * rng.normal(size=n))
+    lam_ref, amp_ref = _modes_ref(L, r, u)
+    scale = np.abs(amp_ref).max()
+    for vec in (False, True):
+        lam = np.zeros(n, complex)
+        c = np.zeros(n, complex)
+        o = np.zeros(n, complex)
+        if vec:
+            info = hc.hc_eig_vec(n, _p(L), _p(lam), _p(r), _p(u), _p(c), _p(o), 30)
+        else:
+            V = np.zeros((n, n), complex)
+            info = hc.hc_eig(n, _p(L), _p(lam), _p(V), _p(r), _p(u), _p(c), _p(o), 30)
+        assert info == 0
+        order = np.lexsort((lam.imag, lam.real))
+        assert np.abs(lam[order] - lam_ref).max() <= 1e-11 * np.abs(lam_ref).max()
+        assert np.abs((c * o)[order] - amp_ref).max() <= 1e-10 * scale
