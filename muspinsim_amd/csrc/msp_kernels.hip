@@ -316,9 +316,13 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 // without spilling, so four 256-thread instances share a CU instead of three.
 // NT = 1024 puts one instance on a whole CU, for the large dimensions whose
 // instances would otherwise leave one wave per SIMD.)
-template <int NT>
+// LDS is a template parameter, not a run-time flag, so that in the LDS form
+// every n-array pointer derives from the LDS block alone and the compiler
+// emits LDS instructions for them (a pointer that may be either LDS or global
+// memory is accessed with flat instructions, slower for LDS).
+template <int NT, bool LDS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
-k_stedc(int n, double *E, double *work, int32_t *info, int small_in_lds)
+k_stedc(int n, double *E, double *work, int32_t *info)
 {
     __shared__ double red[256];
     extern __shared__ __align__(16) unsigned char smem[];
@@ -326,7 +330,7 @@ k_stedc(int n, double *E, double *work, int32_t *info, int small_in_lds)
     double *e;
     cdbl *tau;
     DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
-    if (small_in_lds) {
+    if constexpr (LDS) {
         double *p = (double *)smem;
         W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
         W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
@@ -1051,13 +1055,18 @@ static hipError_t launch_stedc(int n, int batch, double *E, double *work, int32_
     // global memory (n > 512): 256 threads would leave one wave per SIMD
     // (n = 512: 9.8 -> 5.2 ms, n = 1024: 64 -> 28 ms per 140 instances)
     const bool big = !in_lds || 2 * lds2 > 160 * 1024;
-    hipError_t err = msp_set_lds(big ? (const void *)k_stedc<1024> : (const void *)k_stedc<256>, lds2);
+    const void *fn = !in_lds ? (const void *)k_stedc<1024, false>
+                     : big   ? (const void *)k_stedc<1024, true>
+                             : (const void *)k_stedc<256, true>;
+    hipError_t err = msp_set_lds(fn, lds2);
     if (err != hipSuccess) return err;
     ProfScope ps_("k_stedc", s);
-    if (big)
-        hipLaunchKernelGGL(k_stedc<1024>, dim3(batch), dim3(1024), lds2, s, n, E, work, info, in_lds);
+    if (!in_lds)
+        hipLaunchKernelGGL((k_stedc<1024, false>), dim3(batch), dim3(1024), 0, s, n, E, work, info);
+    else if (big)
+        hipLaunchKernelGGL((k_stedc<1024, true>), dim3(batch), dim3(1024), lds2, s, n, E, work, info);
     else
-        hipLaunchKernelGGL(k_stedc<256>, dim3(batch), dim3(256), lds2, s, n, E, work, info, in_lds);
+        hipLaunchKernelGGL((k_stedc<256, true>), dim3(batch), dim3(256), lds2, s, n, E, work, info);
     return hipGetLastError();
 }
 

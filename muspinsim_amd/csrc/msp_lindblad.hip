@@ -73,24 +73,27 @@ __global__ void __launch_bounds__(kNT) k_liouvillian(int n, const double *H, int
 __host__ __device__ inline size_t modes_vec_elems(int N) { return 6 * (size_t)N; }
 
 // NT threads per instance: one wave when the matrix is in LDS (N <= 64: a
-// sweep step's rows and columns are one element per lane, and the barriers of
-// the QR steps are those of a single wave), four above.
-template <int NT>
+// sweep step's rows and columns are one element per lane, and the phase
+// boundaries are those of a single wave), four above.  LDS and VMODE are
+// template parameters so that the LDS form's pointers derive from the LDS
+// block alone and compile to LDS instructions (a pointer that may be LDS or
+// global is accessed with slower flat instructions).  VMODE (LDS, no
+// eigenvectors to return): Z^H r and u_q^T Z are carried through the
+// reduction instead of Z, which halves the LDS.
+template <int NT, bool LDS, bool VMODE>
 __global__ void __launch_bounds__(NT) k_lindblad_modes(int N, double *Lg, const double *r,
                                                        int n_ops, const double *u, double *lam,
                                                        double *amp, double *cout, double *Vout,
-                                                       int32_t *info, double *work, int in_lds,
-                                                       int max_its)
+                                                       int32_t *info, double *work, int max_its)
 {
+    static_assert(LDS || !VMODE, "the vector form is the LDS form");
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[NT];
     const int b = blockIdx.x;
     cdbl *A, *Z, *vec;
     int ld;
-    // without eigenvectors to return, an LDS-resident matrix carries Z^H r and
-    // u_q^T Z through the reduction instead of Z (half the LDS)
-    const bool vmode = in_lds && !Vout;
-    if (in_lds) {
+    constexpr bool vmode = VMODE;
+    if constexpr (LDS) {
         ld = N + 1;
         A = (cdbl *)smem;
         Z = vmode ? nullptr : A + (size_t)N * ld;
@@ -246,16 +249,21 @@ hipError_t launch_lindblad_modes(int N, int batch, double *L, const double *r, i
     const size_t lds = !in_lds ? 0
                        : V ? sizeof(cdbl) * (2 * (size_t)N * (N + 1) + modes_vec_elems(N))
                            : sizeof(cdbl) * ((size_t)N * (N + 1) + modes_vec_elems(N) + (size_t)n_ops * N);
-    const void *fn = in_lds ? (const void *)k_lindblad_modes<64> : (const void *)k_lindblad_modes<kNT>;
+    const void *fn = !in_lds ? (const void *)k_lindblad_modes<kNT, false, false>
+                     : V     ? (const void *)k_lindblad_modes<64, true, false>
+                             : (const void *)k_lindblad_modes<64, true, true>;
     hipError_t err = msp_set_lds(fn, lds);
     if (err != hipSuccess) return err;
     ProfScope ps_("k_lindblad_modes", s);
-    if (in_lds)
-        hipLaunchKernelGGL(k_lindblad_modes<64>, dim3(batch), dim3(64), lds, s, N, L, r, n_ops, u,
-                           lam, amp, c, V, info, work, in_lds, max_its);
+    if (!in_lds)
+        hipLaunchKernelGGL((k_lindblad_modes<kNT, false, false>), dim3(batch), dim3(kNT), lds, s, N, L,
+                           r, n_ops, u, lam, amp, c, V, info, work, max_its);
+    else if (V)
+        hipLaunchKernelGGL((k_lindblad_modes<64, true, false>), dim3(batch), dim3(64), lds, s, N, L, r,
+                           n_ops, u, lam, amp, c, V, info, work, max_its);
     else
-        hipLaunchKernelGGL(k_lindblad_modes<kNT>, dim3(batch), dim3(kNT), lds, s, N, L, r, n_ops, u,
-                           lam, amp, c, V, info, work, in_lds, max_its);
+        hipLaunchKernelGGL((k_lindblad_modes<64, true, true>), dim3(batch), dim3(64), lds, s, N, L, r,
+                           n_ops, u, lam, amp, c, V, info, work, max_its);
     return hipGetLastError();
 }
 
