@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(64 * NW) k_larft_mfma(int n, const double *H, 
 }
 
 template <int RT, int NW>
-__global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backtransform_reg(int n, const double *H, double *work,
+__global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 4 : 1) k_backtransform_reg(int n, const double *H, double *work,
                                                                double *U, int off, int nref,
                                                                int zc)
 {
@@ -180,6 +180,13 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int k0 = blk * MSP_BT_NB;
         const int nbk = (nref - k0 < MSP_BT_NB) ? nref - k0 : MSP_BT_NB;
+        // an opaque zero added to the tile rows keeps the compiler from
+        // hoisting every tile's row offsets out of the block loop (64-bit
+        // addresses for all tiles, live across it: the 8-wave form spilled
+        // 380 bytes per lane and ran n = 1024 at 41.0 ms per 128 matrices, 29.1
+        // without; the 4-wave form fits 128 registers, four workgroups per CU)
+        int zo = 0;
+        asm volatile("" : "+s"(zo));
         // ---- W = V^H U over this wave's tiles (rows >= k0 + off only matter)
         msp_d4 wr = d4zero(), wi = d4zero();
 #pragma unroll
@@ -188,7 +195,7 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
             if (t < ntiles && 16 * t + 15 >= k0 + off) {
                 cdbl v[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + 4 * s + lq, li, off);
+                for (int s = 0; s < 4; ++s) v[s] = vload(n, A, k0, nbk, 16 * t + 4 * s + lq + zo, li, off);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
                     mfma_cmac(v[s].re, -v[s].im, ur[i][s], ui[i][s], wr, wi);   // conj(V)
@@ -243,7 +250,7 @@ __global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 3 : 1) k_backt
         for (int i = 0; i < RT; ++i) {
             const int t = wave + NW * i;
             if (t < ntiles && 16 * t + 15 >= k0 + off) {
-                const int r = 16 * t + li;
+                const int r = 16 * t + li + zo;
                 const int rc = (r < n) ? r : n - 1;
                 cdbl v[4];
 #pragma unroll
