@@ -211,7 +211,9 @@ constexpr int kLB64 = kTT + 2;     // padded row of a staged phase block
 // 64-time tile of one instance (layout and staging as k_timesum_mfma, with
 // 16-deep chunks: four elements per thread for the C block and for each
 // phase block).
-__global__ void __launch_bounds__(256) k_timesum_mfma64(int n, int ntimes, int ldt, const double *C,
+// (three waves per SIMD: 152 VGPRs with the accumulators among them, no
+// spills; the compiler's default form held them in 64 AGPRs, two waves)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_timesum_mfma64(int n, int ntimes, int ldt, const double *C,
                                                         const double *Fc, const double *Fs,
                                                         const double *w, double *out)
 {
