@@ -137,8 +137,12 @@ __global__ void __launch_bounds__(64 * NW) k_larft_mfma(int n, const double *H, 
     }
 }
 
+// Workgroups per CU the register budget is set for: four up to three tiles per
+// wave; two for the 4-wave, 8-tile form (n <= 512: 247 VGPRs, no spills,
+// 11.8 -> 6.9 ms per 256 matrices against one); one otherwise (the 4-tile
+// form measured 4 % slower at two)
 template <int RT, int NW>
-__global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 4 : 1) k_backtransform_reg(int n, const double *H, double *work,
+__global__ void __launch_bounds__(64 * NW, (NW == 4 && RT <= 3) ? 4 : (NW == 4 && RT > 4 ? 2 : 1)) k_backtransform_reg(int n, const double *H, double *work,
                                                                double *U, int off, int nref,
                                                                int zc)
 {
