@@ -316,11 +316,14 @@ __global__ void k_tridiag(int n, double *H, double *E, double *work)
 // without spilling, so four 256-thread instances share a CU instead of three.
 // NT = 1024 puts one instance on a whole CU, for the large dimensions whose
 // instances would otherwise leave one wave per SIMD.)
-// LDS is a template parameter, not a run-time flag, so that in the LDS form
-// every n-array pointer derives from the LDS block alone and the compiler
-// emits LDS instructions for them (a pointer that may be either LDS or global
-// memory is accessed with flat instructions, slower for LDS).
-template <int NT, bool LDS>
+// The placement of the n-arrays is a template parameter, not a run-time flag,
+// so that each LDS pointer derives from the LDS block alone and the compiler
+// emits LDS instructions for it (a pointer that may be either LDS or global
+// memory is accessed with flat instructions, slower for LDS).  MODE 0: all in
+// the global workspace; 1: all in LDS; 2: all in LDS but the four arrays of
+// the deflation rotations (rc, rs, rp, rn), which stay in the workspace, so
+// that n up to 1024 fits one CU's LDS (152 n bytes).
+template <int NT, int MODE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_stedc(int n, double *E, double *work, int32_t *info)
 {
@@ -330,7 +333,7 @@ k_stedc(int n, double *E, double *work, int32_t *info)
     double *e;
     cdbl *tau;
     DcWork W = dc_carve(n, work + (size_t)b * dc_work_doubles(n), &e, &tau);
-    if constexpr (LDS) {
+    if constexpr (MODE == 1) {
         double *p = (double *)smem;
         W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
         W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
@@ -341,12 +344,25 @@ k_stedc(int n, double *E, double *work, int32_t *info)
         W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
         W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
         W.sup = q; q += n; W.kperm = q; q += n; W.nka = q; q += n; W.kb0 = q; q += n;
+    } else if constexpr (MODE == 2) {
+        double *p = (double *)smem;
+        W.dv = p; p += n; W.zv = p; p += n; W.ds = p; p += n; W.zs = p; p += n;
+        W.dl = p; p += n; W.zl = p; p += n; W.zh = p; p += n; W.ktau = p; p += n;
+        W.item_v = p; p += n; W.outv = p; p += n;
+        int *q = (int *)p;
+        W.perm = q; q += n; W.kcol = q; q += n; W.korg = q; q += n; W.defl = q; q += n;
+        W.nkept = q; q += n; W.nrot = q; q += n;
+        W.dfl = q; q += n; W.item_src = q; q += n; W.outsrc = q; q += n; W.rootpos = q; q += n;
+        W.blo = q; q += n; W.bhi = q; q += n; W.plo = q; q += n; W.phi = q; q += n;
+        W.sup = q; q += n; W.kperm = q; q += n; W.nka = q; q += n; W.kb0 = q; q += n;
     }
     stedc_team(n, E + (size_t)b * n, e, W, red);
     if (threadIdx.x == 0) info[b] = 0;
 }
 
 size_t stedc_lds_bytes(int n) { return (size_t)n * (12 * sizeof(double) + 20 * sizeof(int)) + 16; }
+// LDS of MODE 2 (without the rotation arrays)
+static size_t stedc_lds2_bytes(int n) { return (size_t)n * (10 * sizeof(double) + 18 * sizeof(int)) + 16; }
 
 constexpr int kLarftStageMaxDev = 256;   // n up to which k_larft stages the block in LDS
 
@@ -1049,24 +1065,29 @@ static hipError_t launch_stedc(int n, int batch, double *E, double *work, int32_
                                hipStream_t s)
 {
     const int in_lds = (n <= 512) ? 1 : 0;
-    const size_t lds2 = in_lds ? stedc_lds_bytes(n) : 0;
+    // 512 < n <= 1024: LDS but for the rotation arrays (152 n bytes)
+    const bool part = !in_lds && n <= 1024;
+    const size_t lds2 = in_lds ? stedc_lds_bytes(n) : part ? stedc_lds2_bytes(n) : 0;
     // one 1024-thread workgroup per instance where the instance's LDS
     // arrays leave room for one workgroup per CU (n >= 466) or live in
     // global memory (n > 512): 256 threads would leave one wave per SIMD
     // (n = 512: 9.8 -> 5.2 ms, n = 1024: 64 -> 28 ms per 140 instances)
     const bool big = !in_lds || 2 * lds2 > 160 * 1024;
-    const void *fn = !in_lds ? (const void *)k_stedc<1024, false>
-                     : big   ? (const void *)k_stedc<1024, true>
-                             : (const void *)k_stedc<256, true>;
+    const void *fn = part    ? (const void *)k_stedc<1024, 2>
+                     : !in_lds ? (const void *)k_stedc<1024, 0>
+                     : big   ? (const void *)k_stedc<1024, 1>
+                             : (const void *)k_stedc<256, 1>;
     hipError_t err = msp_set_lds(fn, lds2);
     if (err != hipSuccess) return err;
     ProfScope ps_("k_stedc", s);
-    if (!in_lds)
-        hipLaunchKernelGGL((k_stedc<1024, false>), dim3(batch), dim3(1024), 0, s, n, E, work, info);
+    if (part)
+        hipLaunchKernelGGL((k_stedc<1024, 2>), dim3(batch), dim3(1024), lds2, s, n, E, work, info);
+    else if (!in_lds)
+        hipLaunchKernelGGL((k_stedc<1024, 0>), dim3(batch), dim3(1024), 0, s, n, E, work, info);
     else if (big)
-        hipLaunchKernelGGL((k_stedc<1024, true>), dim3(batch), dim3(1024), lds2, s, n, E, work, info);
+        hipLaunchKernelGGL((k_stedc<1024, 1>), dim3(batch), dim3(1024), lds2, s, n, E, work, info);
     else
-        hipLaunchKernelGGL((k_stedc<256, true>), dim3(batch), dim3(256), lds2, s, n, E, work, info);
+        hipLaunchKernelGGL((k_stedc<256, 1>), dim3(batch), dim3(256), lds2, s, n, E, work, info);
     return hipGetLastError();
 }
 
