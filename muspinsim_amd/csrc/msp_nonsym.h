@@ -200,10 +200,39 @@ MSP_HD void hessenberg_team(int n, cdbl *A, int lda, cdbl *Z, int ldz, cdbl *v, 
     }
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates refined by two Newton steps
+// (within an ulp or two of the IEEE result for normal x, a fraction of the
+// instructions of an IEEE division or square root); exact on the host.
+MSP_HD double ns_recip(double x)
+{
+#ifdef MSP_HOST_BUILD
+    return 1.0 / x;
+#else
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#endif
+}
+MSP_HD double ns_rsqrt(double x)
+{
+#ifdef MSP_HOST_BUILD
+    return 1.0 / sqrt(x);
+#else
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+#endif
+}
+
 // Givens rotation G = [c s; -conj(s) c] with G [f; g] = [r; 0], c real:
 // c = |f| / nrm, s = f conj(g) / (|f| nrm), r = f nrm / |f| with
-// nrm = sqrt(|f|^2 + |g|^2), formed from f and g scaled by their largest
-// component (two divisions and two square roots in all).
+// nrm = sqrt(|f|^2 + |g|^2), all from f and g scaled by their largest
+// component and one reciprocal square root, 1 / sqrt(|f|^2 (|f|^2 + |g|^2))
+// (the rotation's step is in the dependent chain of every QR sweep step).
 MSP_HD void givens(cdbl f, cdbl g, double &c, cdbl &s, cdbl &r)
 {
     const double sc = fmax(fmax(fabs(f.re), fabs(f.im)), fmax(fabs(g.re), fabs(g.im)));
@@ -213,22 +242,21 @@ MSP_HD void givens(cdbl f, cdbl g, double &c, cdbl &s, cdbl &r)
         r = f;
         return;
     }
-    const double isc = 1.0 / sc;
+    const double isc = ns_recip(sc);
     const cdbl fs = cscale(f, isc), gs = cscale(g, isc);
     const double f2 = fs.re * fs.re + fs.im * fs.im, g2 = gs.re * gs.re + gs.im * gs.im;
-    if (f.re == 0.0 && f.im == 0.0) {
-        const double ga = sqrt(g2);
-        const double ig = 1.0 / ga;
+    if (f2 < 1e-300) {                                 // f = 0, or |f| < 1e-150 |g|
+        const double ig = ns_rsqrt(g2);
         c = 0.0;
         s = cmk(gs.re * ig, -gs.im * ig);
-        r = cmk(ga * sc, 0.0);
+        r = cmk(g2 * ig * sc, 0.0);
         return;
     }
-    const double fa = sqrt(f2), nrm = sqrt(f2 + g2);
-    const double inv = 1.0 / (fa * nrm);
+    const double t = f2 + g2;
+    const double inv = ns_rsqrt(f2 * t);               // 1 / (|f| nrm), scaled
     c = f2 * inv;                                      // |f| / nrm
     s = cscale(cmul(fs, cconj(gs)), inv);
-    r = cscale(f, (f2 + g2) * inv);                    // f nrm / |f|
+    r = cscale(f, t * inv);                            // f nrm / |f|
 }
 
 // ---------------------------------------------------------------------------
