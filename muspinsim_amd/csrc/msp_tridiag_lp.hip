@@ -116,6 +116,27 @@ __device__ inline void store_colpart(int n, int c0, int wave, bool writer, const
     }
 }
 
+#ifndef MSP_TD_QSTEP
+#define MSP_TD_QSTEP 2  // slot-count granularity of the 16-lane-row passes (1 and 4 measured slower)
+#endif
+
+// rows16_pass with the smallest QA = Q16 - MSP_TD_QSTEP m >= qact
+// (qact wave-uniform), so that a pass loads no slot wholly past its rows.
+template <int W, int Q16, int QA, int RP>
+__device__ __forceinline__ void rows16_dispatch(int qact, int n, int k, int ib, int g, int li,
+                                                const cdbl *A, const cdbl *vb, cdbl *pb,
+                                                cdbl *nxt, cdbl (&colacc)[Q16])
+{
+    constexpr int S = MSP_TD_QSTEP;
+    if constexpr (QA > S) {
+        if (qact <= QA - S) {
+            rows16_dispatch<W, Q16, QA - S, RP>(qact, n, k, ib, g, li, A, vb, pb, nxt, colacc);
+            return;
+        }
+    }
+    rows16_pass<W, Q16, QA, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+}
+
 // A22 v for step k with one matrix row per 16-lane group: wave w takes rows
 // k+1 + 4 W t + 4 w + g (g = lane / 16), a lane the columns k+1 + (lane % 16)
 // + 16 q.  A row sum is a 16-lane DPP reduction shared by the wave's four
@@ -136,11 +157,7 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
         const int ilast0 = ib + 4 * W * (RP - 1) + 3;
         const int ilast = (ilast0 < n) ? ilast0 : n - 1;
         const int qact = ((ilast - (k + 1)) >> 4) + 1;          // wave-uniform, <= Q16
-        // slot counts in steps of 4 keep the number of code paths small
-        if (qact <= 4) rows16_pass<W, Q16, 4, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
-        else if (Q16 >= 8 && qact <= 8)
-            rows16_pass<W, Q16, (Q16 >= 8 ? 8 : 4), RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
-        else rows16_pass<W, Q16, Q16, RP>(n, k, ib, g, li, A, vb, pb, nxt, colacc);
+        rows16_dispatch<W, Q16, Q16, RP>(qact, n, k, ib, g, li, A, vb, pb, nxt, colacc);
     }
     const int qstep = ((n - 1 - (k + 1)) >> 4) + 1;
 #pragma unroll
@@ -163,11 +180,18 @@ __device__ inline void matvec_rows16(int n, int k, int wave, int lane, const cdb
 // loads of a pass issued first); a row's Vp and Wp entries are wave-uniform
 // LDS reads.  A(i, c) -= sum_l Vp_il conj(Wp_cl) + Wp_il conj(Vp_cl), c <= i.
 // (always_inline: as a call, the ABI's saved registers spill to scratch)
-template <int W, int NB, int RB>
+template <int W, int NB, int RB, bool GP>
 __device__ __attribute__((always_inline)) inline void panel_update_cols(
-    int n, int t0, int wave, int lane, cdbl *A, const cdbl *Vp, const cdbl *Wp, cdbl *col)
+    int n, int t0, int wave, int lane, cdbl *A, const cdbl *Vp, const cdbl *Wp, cdbl *col,
+    cdbl *stage)
 {
     constexpr int LDP = NB + 1;
+    // With the panel pair in global memory (GP), a pass's 2 NB panel entries
+    // of each of its RB rows are fetched by one load per lane, issued with
+    // the pass's matrix loads, and read back from the wave's LDS block, so
+    // no row waits on a global round trip of its own.
+    static_assert(!GP || 2 * NB * RB == 64, "one staged panel entry per lane");
+    cdbl *stw = stage + 64 * wave;
     for (int cb = t0; cb < n; cb += 64) {
         const int c = cb + lane;
         const bool cok = c < n;
@@ -186,6 +210,16 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
                 const int ic = (i < n) ? i : n - 1;
                 av[u] = A[(size_t)ic * n + ((cc <= ic) ? cc : ic)];
             }
+            if constexpr (GP) {
+                // lane = 2 NB u + m: row i0 + u W, Vp column m (m < NB) or
+                // Wp column m - NB
+                const int u = lane / (2 * NB), m = lane % (2 * NB);
+                const int i = i0 + u * W;
+                const int ic = (i < n) ? i : n - 1;
+                const cdbl pv = (m < NB) ? Vp[ic * LDP + m] : Wp[ic * LDP + m - NB];
+                stw[lane] = pv;
+                __builtin_amdgcn_wave_barrier();
+            }
 #pragma unroll
             for (int u = 0; u < RB; ++u) {
                 const int i = i0 + u * W;
@@ -193,16 +227,92 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
                 cdbl a = av[u];
 #pragma unroll
                 for (int l = 0; l < NB; ++l) {
-                    a = csub(a, cmul(Vp[i * LDP + l], wc[l]));
-                    a = csub(a, cmul(Wp[i * LDP + l], vc[l]));
+                    const cdbl vp = GP ? stw[2 * NB * u + l] : Vp[i * LDP + l];
+                    const cdbl wp = GP ? stw[2 * NB * u + NB + l] : Wp[i * LDP + l];
+                    a = csub(a, cmul(vp, wc[l]));
+                    a = csub(a, cmul(wp, vc[l]));
                 }
                 if (cok && c <= i) {
                     A[(size_t)i * n + c] = a;
                     if (c == t0) col[i] = a;             // next step's column
                 }
             }
+            if constexpr (GP) __builtin_amdgcn_wave_barrier();
         }
     }
+}
+
+// One pass of the 64-lane-row A22 v product over the rows i0 + u W (u < RB)
+// of step k, with the first QN column slots (columns k+1 + lane + 64 q,
+// q < QN).  QN is a compile-time constant, so every load of the pass is
+// issued, from clamped addresses with the value masked, before the first
+// use; the caller picks the smallest QN that reaches the pass's longest row.
+template <int W, int QMAX, int RB, int QN>
+__device__ __forceinline__ void a22v_pass(int n, int k, int i0, int lane, const cdbl *A,
+                                          const cdbl *vb, cdbl *pb, cdbl *nxt,
+                                          cdbl (&colacc)[QMAX])
+{
+    cdbl av[RB][QN];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+        const int i = i0 + u * W;
+        const int ic = (i < n) ? i : n - 1;
+        const cdbl *row = A + (size_t)ic * n;
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+            const int c = k + 1 + lane + 64 * q;
+            const int cc = (c <= ic) ? c : ic;
+            av[u][q] = row[cc];
+        }
+    }
+    double rr[RB], ri[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+        const int i = i0 + u * W;
+        const bool rowok = i < n;
+        const cdbl vi = vb[rowok ? i : n - 1];
+        rr[u] = 0.0;
+        ri[u] = 0.0;
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+            const int c = k + 1 + lane + 64 * q;
+            if (rowok && c <= i) {
+                const cdbl a = av[u][q];
+                if (q == 0 && lane == 0) nxt[i] = a;   // column k+1
+                const cdbl vc = vb[c];
+                rr[u] += a.re * vc.re - a.im * vc.im;
+                ri[u] += a.re * vc.im + a.im * vc.re;
+                if (c < i) cfmac(colacc[q], a, vi);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+        rr[u] = wave_sum_to63(rr[u]);
+        ri[u] = wave_sum_to63(ri[u]);
+    }
+    if (lane == 63) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int i = i0 + u * W;
+            if (i < n) pb[i] = cmk(rr[u], ri[u]);
+        }
+    }
+}
+
+// a22v_pass with QN = qn, 1 <= qn <= Q (qn wave-uniform).
+template <int W, int QMAX, int RB, int Q>
+__device__ __forceinline__ void a22v_pass_q(int qn, int n, int k, int i0, int lane, const cdbl *A,
+                                            const cdbl *vb, cdbl *pb, cdbl *nxt,
+                                            cdbl (&colacc)[QMAX])
+{
+    if constexpr (Q > 1) {
+        if (qn < Q) {
+            a22v_pass_q<W, QMAX, RB, Q - 1>(qn, n, k, i0, lane, A, vb, pb, nxt, colacc);
+            return;
+        }
+    }
+    a22v_pass<W, QMAX, RB, Q>(n, k, i0, lane, A, vb, pb, nxt, colacc);
 }
 
 template <int W, int NB, int QMAX, int RB, int Q16, bool GP>
@@ -231,6 +341,7 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
     cdbl *colA = y + 2 * NB;                         // n: current column k
     cdbl *colB = colA + n;                           // n: column k+1, stashed by the matvec
     double *red = (double *)(colB + n);              // 2 W
+    cdbl *stage = (cdbl *)(red + 2 * W);             // GP: 64 per wave
 
     cdbl *col = colA, *nxt = colB;
     for (int i = tid; i < n; i += kThreads) col[i] = A[(size_t)i * n];
@@ -315,52 +426,11 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
                     // the pass is issued first, from clamped addresses with the
                     // value masked, so no branch separates the loads
                     for (int i0 = k + 1 + wave; i0 < n; i0 += RB * W) {
-                        cdbl av[RB][QMAX];
-    #pragma unroll
-                        for (int u = 0; u < RB; ++u) {
-                            const int i = i0 + u * W;
-                            const int ic = (i < n) ? i : n - 1;
-                            const cdbl *row = A + (size_t)ic * n;
-    #pragma unroll
-                            for (int q = 0; q < QMAX; ++q) {
-                                const int c = k + 1 + lane + 64 * q;
-                                const int cc = (c <= ic) ? c : ic;
-                                av[u][q] = row[cc];
-                            }
-                        }
-                        double rr[RB], ri[RB];
-    #pragma unroll
-                        for (int u = 0; u < RB; ++u) {
-                            const int i = i0 + u * W;
-                            const bool rowok = i < n;
-                            const cdbl vi = vb[rowok ? i : n - 1];
-                            rr[u] = 0.0;
-                            ri[u] = 0.0;
-    #pragma unroll
-                            for (int q = 0; q < QMAX; ++q) {
-                                const int c = k + 1 + lane + 64 * q;
-                                if (rowok && c <= i) {
-                                    const cdbl a = av[u][q];
-                                    if (q == 0 && lane == 0) nxt[i] = a;   // column k+1
-                                    const cdbl vc = vb[c];
-                                    rr[u] += a.re * vc.re - a.im * vc.im;
-                                    ri[u] += a.re * vc.im + a.im * vc.re;
-                                    if (c < i) cfmac(colacc[q], a, vi);
-                                }
-                            }
-                        }
-    #pragma unroll
-                        for (int u = 0; u < RB; ++u) {
-                            rr[u] = wave_sum_to63(rr[u]);
-                            ri[u] = wave_sum_to63(ri[u]);
-                        }
-                        if (lane == 63) {
-    #pragma unroll
-                            for (int u = 0; u < RB; ++u) {
-                                const int i = i0 + u * W;
-                                if (i < n) pb[i] = cmk(rr[u], ri[u]);
-                            }
-                        }
+                        // slots past the pass's longest row hold c > i for
+                        // every lane and row: neither loaded nor used
+                        const int ilast = (i0 + (RB - 1) * W < n) ? i0 + (RB - 1) * W : n - 1;
+                        const int qn = __builtin_amdgcn_readfirstlane(((ilast - k - 1) >> 6) + 1);
+                        a22v_pass_q<W, QMAX, RB, QMAX>(qn, n, k, i0, lane, A, vb, pb, nxt, colacc);
                     }
                     store_colpart<W, QMAX, 64>(n, k + 1 + lane, wave, true, colacc, cp);
                 }
@@ -420,24 +490,25 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
             const int i = idx / nbk, l = idx % nbk;
             if (i >= k0 + l + 2) A[(size_t)i * n + k0 + l] = Vp[i * LDP + l];
         }
-        panel_update_cols<W, NB, kPanelRows>(n, k0 + nbk, wave, lane, A, Vp, Wp, col);
+        panel_update_cols<W, NB, GP ? 64 / (2 * NB) : kPanelRows, GP>(n, k0 + nbk, wave, lane, A, Vp, Wp,
+                                                                     col, stage);
         __syncthreads();
     }
     if (tid == 0) d[n - 1] = col[n - 1].re;
 }
 
-template <int NB, bool GP>
+template <int W, int NB, bool GP>
 size_t lp_lds(int n)
 {
     return sizeof(cdbl) * ((GP ? 0 : (size_t)2 * n * (NB + 1)) + 4 * (size_t)n +
-                           (size_t)kCpRows * n + 2 * NB) +
+                           (size_t)kCpRows * n + 2 * NB + (GP ? 64 * W : 0)) +
            16 * sizeof(double);
 }
 
 template <int W, int NB, int QMAX, int RB, int Q16, bool GP = false>
 hipError_t lp_launch(int n, int batch, double *H, double *E, double *work, hipStream_t s)
 {
-    size_t lds = lp_lds<NB, GP>(n);
+    size_t lds = lp_lds<W, NB, GP>(n);
     hipError_t err = msp_set_lds((const void *)k_tridiag_lp<W, NB, QMAX, RB, Q16, GP>, lds);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL((k_tridiag_lp<W, NB, QMAX, RB, Q16, GP>), dim3(batch), dim3(64 * W), lds, s, n,
