@@ -33,6 +33,12 @@ constexpr int kPanelRows = 8;   // rows per batched pass of the end-of-panel upd
 #ifndef MSP_TD_WL
 #define MSP_TD_WL 8     // waves per workgroup above n = 192
 #endif
+#ifndef MSP_TD_RBMAX
+#define MSP_TD_RBMAX 4  // most rows in one pass of the 64-lane-row A22 v product
+#endif
+#ifndef MSP_TD_NBL
+#define MSP_TD_NBL 8    // panel width above n = 512 (panels in the workspace; 4 was 5-13 % slower)
+#endif
 #ifndef MSP_TD_RB1024
 #define MSP_TD_RB1024 1 // rows per pass of the A22 v product for n <= 1024 (2 spills at 8 waves)
 #endif
@@ -315,6 +321,38 @@ __device__ __forceinline__ void a22v_pass_q(int qn, int n, int k, int i0, int la
     a22v_pass<W, QMAX, RB, Q>(n, k, i0, lane, A, vb, pb, nxt, colacc);
 }
 
+// One pass of the A22 v product at row i0, returning the rows it covered.
+// Passes whose rows are short (few column slots) take 2 RB or 4 RB rows
+// within the same slot budget SB = RB QMAX, so a wave keeps as many loads in
+// flight near the diagonal as in the full-width passes.
+template <int W, int QMAX, int RB>
+__device__ __forceinline__ int a22v_rows(int n, int k, int i0, int lane, const cdbl *A,
+                                         const cdbl *vb, cdbl *pb, cdbl *nxt,
+                                         cdbl (&colacc)[QMAX])
+{
+    constexpr int SB = RB * QMAX;
+    auto slots = [&](int rows) {
+        const int il = (i0 + (rows - 1) * W < n) ? i0 + (rows - 1) * W : n - 1;
+        return __builtin_amdgcn_readfirstlane(((il - k - 1) >> 6) + 1);
+    };
+    if constexpr (SB / (4 * RB) >= 1 && 4 * RB <= MSP_TD_RBMAX) {
+        const int qn = slots(4 * RB);
+        if (qn <= SB / (4 * RB)) {
+            a22v_pass_q<W, QMAX, 4 * RB, SB / (4 * RB)>(qn, n, k, i0, lane, A, vb, pb, nxt, colacc);
+            return 4 * RB * W;
+        }
+    }
+    if constexpr (SB / (2 * RB) >= 1 && 2 * RB <= MSP_TD_RBMAX) {
+        const int qn = slots(2 * RB);
+        if (qn <= SB / (2 * RB)) {
+            a22v_pass_q<W, QMAX, 2 * RB, SB / (2 * RB)>(qn, n, k, i0, lane, A, vb, pb, nxt, colacc);
+            return 2 * RB * W;
+        }
+    }
+    a22v_pass_q<W, QMAX, RB, QMAX>(slots(RB), n, k, i0, lane, A, vb, pb, nxt, colacc);
+    return RB * W;
+}
+
 template <int W, int NB, int QMAX, int RB, int Q16, bool GP>
 __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double *E, double *work)
 {
@@ -425,13 +463,10 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
                     // RB rows per pass (i, i+W, ... of this wave): every load of
                     // the pass is issued first, from clamped addresses with the
                     // value masked, so no branch separates the loads
-                    for (int i0 = k + 1 + wave; i0 < n; i0 += RB * W) {
-                        // slots past the pass's longest row hold c > i for
-                        // every lane and row: neither loaded nor used
-                        const int ilast = (i0 + (RB - 1) * W < n) ? i0 + (RB - 1) * W : n - 1;
-                        const int qn = __builtin_amdgcn_readfirstlane(((ilast - k - 1) >> 6) + 1);
-                        a22v_pass_q<W, QMAX, RB, QMAX>(qn, n, k, i0, lane, A, vb, pb, nxt, colacc);
-                    }
+                    // (slots past a pass's longest row hold c > i for every
+                    // lane and row: neither loaded nor used)
+                    for (int i0 = k + 1 + wave; i0 < n;)
+                        i0 += a22v_rows<W, QMAX, RB>(n, k, i0, lane, A, vb, pb, nxt, colacc);
                     store_colpart<W, QMAX, 64>(n, k + 1 + lane, wave, true, colacc, cp);
                 }
                 __syncthreads();
@@ -530,6 +565,6 @@ hipError_t launch_tridiag_lp(int n, int batch, double *H, double *E, double *wor
     // above n = 192 one workgroup fills a CU's LDS: 8 waves put two on each SIMD
     if (n <= 256) return lp_launch<MSP_TD_WL, 8, 4, 4, 0>(n, batch, H, E, work, s);
     if (n <= 512) return lp_launch<MSP_TD_WL, 4, 8, 4, 0>(n, batch, H, E, work, s);
-    if (n <= 1024) return lp_launch<MSP_TD_WL, 4, 16, MSP_TD_RB1024, 0, true>(n, batch, H, E, work, s);
+    if (n <= 1024) return lp_launch<MSP_TD_WL, MSP_TD_NBL, 16, MSP_TD_RB1024, 0, true>(n, batch, H, E, work, s);
     return hipErrorNotSupported;
 }
