@@ -256,9 +256,11 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
     org_out = org;
     if (K == 1) return zz[0];
     const double dorg = dl[org];
-    // interior model uses the two poles j and j+1 around the root
-    const int pl = (j < K - 1) ? j : K - 1;
-    const int pr = (j < K - 1) ? j + 1 : -1;
+    // the model keeps two poles exact: j and j+1 around an interior root, the
+    // last two poles for the last root (right of both), the other poles of
+    // each side lumped into one rational term
+    const int pl = interior ? j : K - 2;
+    const int pr = interior ? j + 1 : K - 1;
     double tau = interior ? m_tau : 0.5 * (lo + hi);
     for (int it = 0; it < 80; ++it) {
         // psi / phi: poles left / right of the root; with r = 1/del,
@@ -285,25 +287,33 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
         // bisection)
         if (fabs(w) <= 8.0 * eps * K * (1.0 + fabs(psi) + fabs(phi))) break;
         if (w < 0.0) lo = tau; else hi = tau;
+        // model c + Dl^2 psi'/(Dl - eta) + Dr^2 phi'/(Dr - eta) (value and
+        // slope of each side matched at tau) = 0, i.e. c eta^2 - a eta + b = 0
         double eta;
         const double Dl = (dl[pl] - dorg) - tau;
-        if (pr >= 0) {
-            const double Dr = (dl[pr] - dorg) - tau;
-            double c = w - Dl * dpsi - Dr * dphi;
-            double a = (Dl + Dr) * w - Dl * Dr * (dpsi + dphi);
-            double b = Dl * Dr * w;
-            double disc = sqrt(fabs(a * a - 4.0 * b * c));
+        const double Dr = (dl[pr] - dorg) - tau;
+        double c = w - Dl * dpsi - Dr * dphi;
+        const double a = (Dl + Dr) * w - Dl * Dr * (dpsi + dphi);
+        const double b = Dl * Dr * w;
+        if (interior) {
+            // the root between the two poles
+            const double disc = sqrt(fabs(a * a - 4.0 * b * c));
             if (c == 0.0) eta = b / a;
             else if (a <= 0.0) eta = (a - disc) / (2.0 * c);
             else eta = 2.0 * b / (a + disc);
         } else {
-            // single pole on the left: c + s/(Dl - eta) = 0, s = Dl^2 psi'
-            double c = w - Dl * dpsi;
-            double s = Dl * Dl * dpsi;
-            eta = (c != 0.0) ? Dl + s / c : 0.5 * (lo + hi) - tau;
+            // the root right of both poles: f -> 1 > 0 there, so the lumped
+            // constant is taken positive and the larger root is used; a step
+            // against the sign of w falls back to Newton's
+            c = fabs(c);
+            const double disc = sqrt(fabs(a * a - 4.0 * b * c));
+            if (c == 0.0) eta = b / a;
+            else if (a >= 0.0) eta = (a + disc) / (2.0 * c);
+            else eta = 2.0 * b / (a - disc);
+            if (w * eta > 0.0) eta = -w / (dpsi + dphi);
         }
         double tn = tau + eta;
-        bool inside = (pr >= 0) ? (tn > lo && tn < hi) : (tn > lo && tn <= hi);
+        bool inside = interior ? (tn > lo && tn < hi) : (tn > lo && tn <= hi);
         if (!inside) tn = 0.5 * (lo + hi);
         double step = fabs(tn - tau);
         tau = tn;
