@@ -287,14 +287,28 @@ MSP_HD double secular_root(int K, const double *dl, const double *zl, const doub
         // bisection)
         if (fabs(w) <= 8.0 * eps * K * (1.0 + fabs(psi) + fabs(phi))) break;
         if (w < 0.0) lo = tau; else hi = tau;
-        // model c + Dl^2 psi'/(Dl - eta) + Dr^2 phi'/(Dr - eta) (value and
-        // slope of each side matched at tau) = 0, i.e. c eta^2 - a eta + b = 0
+        // model c + sl/(Dl - eta) + sr/(Dr - eta) = 0, i.e. c eta^2 - a eta
+        // + b = 0 with a = c (Dl + Dr) + sl + sr, b = c Dl Dr + sl Dr + sr Dl
+        // and c = w - sl/Dl - sr/Dr.  The first step keeps the two poles'
+        // own weights (sl, sr = rho z^2; every other pole goes into the
+        // constant), which lands near a root that sits close to a pole of
+        // small weight; later steps match each side's slope at tau (sl =
+        // Dl^2 psi', sr = Dr^2 phi').
         double eta;
         const double Dl = (dl[pl] - dorg) - tau;
         const double Dr = (dl[pr] - dorg) - tau;
+        double sl = Dl * Dl * dpsi, sr = Dr * Dr * dphi;
         double c = w - Dl * dpsi - Dr * dphi;
-        const double a = (Dl + Dr) * w - Dl * Dr * (dpsi + dphi);
-        const double b = Dl * Dr * w;
+        if (it == 0) {
+            const double cf = w - zz[pl] / Dl - zz[pr] / Dr;
+            if (interior || cf > 0.0) {
+                sl = zz[pl];
+                sr = zz[pr];
+                c = cf;
+            }
+        }
+        const double a = c * (Dl + Dr) + sl + sr;
+        const double b = c * Dl * Dr + sl * Dr + sr * Dl;
         if (interior) {
             // the root between the two poles
             const double disc = sqrt(fabs(a * a - 4.0 * b * c));
