@@ -34,7 +34,8 @@ def _dev(a, dt=torch.complex128):
 
 @pytest.mark.parametrize("n,method", [(1, 1), (2, 1), (3, 1), (8, 1), (16, 1), (33, 1), (64, 1),
                                       (2, 2), (5, 2), (16, 2), (65, 2), (128, 2), (192, 2),
-                                      (256, 2), (300, 2), (600, 2), (1000, 2), (1100, 2),
+                                      (161, 2), (256, 2), (300, 2), (513, 2), (600, 2), (767, 2), (1000, 2),
+                                      (1100, 2),
                                       (34, 3), (47, 3), (100, 3), (129, 3), (192, 3), (255, 3),
                                       (256, 3)])
 def test_eigh_random(eng, n, method):
