@@ -426,8 +426,15 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
             } else {
                 double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
                 beta = alpha.re >= 0.0 ? -nrm : nrm;
-                tk = cmk((beta - alpha.re) / beta, -alpha.im / beta);
-                scale = cdivc(cmk(1.0, 0.0), csub(alpha, cmk(beta, 0.0)));
+                // two Newton-refined reciprocals instead of IEEE divisions
+                // (this chain is serial in every column step):
+                // tau = (beta - alpha)/beta, 1/(alpha - beta) = (1/beta)/d
+                // with d = alpha/beta - 1, 1 <= |d| <= 2
+                const double ib = msp_recip(beta);
+                tk = cmk((beta - alpha.re) * ib, -alpha.im * ib);
+                const cdbl dd = cmk(alpha.re * ib - 1.0, alpha.im * ib);
+                const double id = ib * msp_recip(dd.re * dd.re + dd.im * dd.im);
+                scale = cmk(dd.re * id, -dd.im * id);
             }
             const bool act = (tk.re != 0.0 || tk.im != 0.0);
             for (int i = k + 1 + tid; i < n; i += kThreads) {
