@@ -205,12 +205,14 @@ __global__ void __launch_bounds__(kT) k_tridiag_rs(int n, double *H, double *E, 
             } else {
                 const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
                 beta = alpha.re >= 0.0 ? -nrm : nrm;
-                const double rb = 1.0 / beta;
+                // Newton-refined reciprocals (serial in every column step):
+                // tau = (beta - alpha)/beta, 1/(alpha - beta) = (1/beta)/d with
+                // d = alpha/beta - 1, 1 <= |d| <= 2 (beta has the sign opposite
+                // to Re alpha)
+                const double rb = msp_recip(beta);
                 tk = cmk((beta - alpha.re) * rb, -alpha.im * rb);
-                // 1 / (alpha - beta) = conj(alpha - beta) / |alpha - beta|^2
-                // (|alpha - beta| >= |beta| > 0: beta has the sign opposite to Re alpha)
-                const double dr = alpha.re - beta, di = alpha.im;
-                const double rd = 1.0 / (dr * dr + di * di);
+                const double dr = alpha.re * rb - 1.0, di = alpha.im * rb;
+                const double rd = rb * msp_recip(dr * dr + di * di);
                 scale = cmk(dr * rd, -di * rd);
             }
             const bool act = (tk.re != 0.0 || tk.im != 0.0);
