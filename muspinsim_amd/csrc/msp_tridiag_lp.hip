@@ -58,22 +58,28 @@ __device__ inline double row16_sum(double v)
 
 // One pass of the 16-lane-row A22 v product over rows ib + g + 4 W r
 // (r < RP), with the first QA column slots (QA is a compile-time constant so
-// that every load of the pass is issued before the first use).
+// that every load of the pass is issued before the first use).  Slots past a
+// row's diagonal, and rows past n, load A[0][n-1] instead, an element of the
+// strict upper triangle that the kernel zeroes at its start and never writes
+// again, so no loaded value needs a mask.  Every column sum then also takes
+// its diagonal term conj(a_ii) v_i, which the row sum gives back.
 template <int W, int Q16, int QA, int RP>
 __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cdbl *A,
                                    const cdbl *vb, cdbl *pb, cdbl *nxt, cdbl (&colacc)[Q16])
 {
     cdbl av[RP][QA];
+    cdbl dg[RP];
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
         const int i = ib + g + 4 * W * r;
-        const int ic = (i < n) ? i : n - 1;
-        const cdbl *row = A + (size_t)ic * n;
+        const bool rowok = i < n;
+        const int ro = i * n;
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
             const int c = k + 1 + li + 16 * q;
-            av[r][q] = row[(c <= ic) ? c : ic];
+            av[r][q] = A[(rowok && c <= i) ? ro + c : n - 1];
         }
+        dg[r] = A[rowok ? ro + i : n - 1];
     }
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
@@ -84,18 +90,19 @@ __device__ inline void rows16_pass(int n, int k, int ib, int g, int li, const cd
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
             const int c = k + 1 + li + 16 * q;
-            const bool ok = rowok && c <= i;
-            const cdbl a = ok ? av[r][q] : cmk(0.0, 0.0);
+            const cdbl a = av[r][q];
             const cdbl vc = vb[(c < n) ? c : n - 1];
             rr += a.re * vc.re - a.im * vc.im;
             ri += a.re * vc.im + a.im * vc.re;
-            const cdbl vm = (c < i) ? vi : cmk(0.0, 0.0);
-            cfmac(colacc[q], a, vm);
-            if (q == 0 && li == 0 && rowok) nxt[i] = av[r][0];   // column k+1
+            cfmac(colacc[q], a, vi);
+            if (q == 0 && li == 0 && rowok) nxt[i] = a;   // column k+1
         }
         rr = row16_sum(rr);
         ri = row16_sum(ri);
-        if (li == 0 && rowok) pb[i] = cmk(rr, ri);
+        if (li == 0 && rowok) {
+            const cdbl d = dg[r];
+            pb[i] = cmk(rr - (d.re * vi.re + d.im * vi.im), ri - (d.re * vi.im - d.im * vi.re));
+        }
     }
 }
 
@@ -382,6 +389,10 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
     cdbl *stage = (cdbl *)(red + 2 * W);             // GP: 64 per wave
 
     cdbl *col = colA, *nxt = colB;
+    // A[0][n-1] (strict upper triangle, never written below) is the zero that
+    // the 16-lane-row A22 v passes load in place of elements past a row's
+    // diagonal; the first barrier below publishes it
+    if (Q16 > 0 && tid == 0) A[n - 1] = cmk(0.0, 0.0);
     for (int i = tid; i < n; i += kThreads) col[i] = A[(size_t)i * n];
     for (int k0 = 0; k0 < n - 1; k0 += NB) {
         const int nbk = (n - 1 - k0 < NB) ? n - 1 - k0 : NB;
