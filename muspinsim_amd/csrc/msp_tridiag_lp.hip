@@ -36,6 +36,9 @@ constexpr int kPanelRows = 8;   // rows per batched pass of the end-of-panel upd
 #ifndef MSP_TD_RBMAX
 #define MSP_TD_RBMAX 4  // most rows in one pass of the 64-lane-row A22 v product
 #endif
+#ifndef MSP_TD_SPL
+#define MSP_TD_SPL 2    // staged panel entries per lane in the workspace-panel update (1: 2-3 % slower)
+#endif
 #ifndef MSP_TD_NBL
 #define MSP_TD_NBL 8    // panel width above n = 512 (panels in the workspace; 4 was 5-13 % slower)
 #endif
@@ -203,8 +206,8 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
     // of each of its RB rows are fetched by one load per lane, issued with
     // the pass's matrix loads, and read back from the wave's LDS block, so
     // no row waits on a global round trip of its own.
-    static_assert(!GP || 2 * NB * RB == 64, "one staged panel entry per lane");
-    cdbl *stw = stage + 64 * wave;
+    static_assert(!GP || 2 * NB * RB == 64 * MSP_TD_SPL, "MSP_TD_SPL staged panel entries per lane");
+    cdbl *stw = stage + 64 * MSP_TD_SPL * wave;
     for (int cb = t0; cb < n; cb += 64) {
         const int c = cb + lane;
         const bool cok = c < n;
@@ -224,13 +227,19 @@ __device__ __attribute__((always_inline)) inline void panel_update_cols(
                 av[u] = A[(size_t)ic * n + ((cc <= ic) ? cc : ic)];
             }
             if constexpr (GP) {
-                // lane = 2 NB u + m: row i0 + u W, Vp column m (m < NB) or
-                // Wp column m - NB
-                const int u = lane / (2 * NB), m = lane % (2 * NB);
-                const int i = i0 + u * W;
-                const int ic = (i < n) ? i : n - 1;
-                const cdbl pv = (m < NB) ? Vp[ic * LDP + m] : Wp[ic * LDP + m - NB];
-                stw[lane] = pv;
+                // entry e = 2 NB u + m: row i0 + u W, Vp column m (m < NB)
+                // or Wp column m - NB
+                cdbl pv[MSP_TD_SPL];
+#pragma unroll
+                for (int h = 0; h < MSP_TD_SPL; ++h) {
+                    const int e = lane + 64 * h;
+                    const int u = e / (2 * NB), m = e % (2 * NB);
+                    const int i = i0 + u * W;
+                    const int ic = (i < n) ? i : n - 1;
+                    pv[h] = (m < NB) ? Vp[ic * LDP + m] : Wp[ic * LDP + m - NB];
+                }
+#pragma unroll
+                for (int h = 0; h < MSP_TD_SPL; ++h) stw[lane + 64 * h] = pv[h];
                 __builtin_amdgcn_wave_barrier();
             }
 #pragma unroll
@@ -386,7 +395,7 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
     cdbl *colA = y + 2 * NB;                         // n: current column k
     cdbl *colB = colA + n;                           // n: column k+1, stashed by the matvec
     double *red = (double *)(colB + n);              // 2 W
-    cdbl *stage = (cdbl *)(red + 2 * W);             // GP: 64 per wave
+    cdbl *stage = (cdbl *)(red + 2 * W);             // GP: 64 MSP_TD_SPL per wave
 
     cdbl *col = colA, *nxt = colB;
     // A[0][n-1] (strict upper triangle, never written below) is the zero that
@@ -543,7 +552,7 @@ __global__ void __launch_bounds__(64 * W) k_tridiag_lp(int n, double *H, double 
             const int i = idx / nbk, l = idx % nbk;
             if (i >= k0 + l + 2) A[(size_t)i * n + k0 + l] = Vp[i * LDP + l];
         }
-        panel_update_cols<W, NB, GP ? 64 / (2 * NB) : kPanelRows, GP>(n, k0 + nbk, wave, lane, A, Vp, Wp,
+        panel_update_cols<W, NB, GP ? 64 * MSP_TD_SPL / (2 * NB) : kPanelRows, GP>(n, k0 + nbk, wave, lane, A, Vp, Wp,
                                                                      col, stage);
         __syncthreads();
     }
@@ -554,7 +563,7 @@ template <int W, int NB, bool GP>
 size_t lp_lds(int n)
 {
     return sizeof(cdbl) * ((GP ? 0 : (size_t)2 * n * (NB + 1)) + 4 * (size_t)n +
-                           (size_t)kCpRows * n + 2 * NB + (GP ? 64 * W : 0)) +
+                           (size_t)kCpRows * n + 2 * NB + (GP ? 64 * MSP_TD_SPL * W : 0)) +
            16 * sizeof(double);
 }
 
